@@ -1,0 +1,136 @@
+/*
+ * nomad_hip.h -- C ABI of the MI355X NOMAD SGD library (libnomad_hip.so).
+ *
+ * The reference exposes no library interface for its hot path: the SGD loop is
+ * the private member NomadBody::updater_func (Code/nomad/nomad_body.h:214,
+ * Code/nomad/nb_main.cpp:415-710), driven by NomadBody::run
+ * (Code/nomad/nomad_body.h:278, Code/nomad/nb_main.cpp:30-410).  This header is
+ * the boundary that replaces the pieces of that member a host driver needs, as
+ * plain C with pointers and sizes:
+ *
+ *   nomad_create        NomadBody::initial + Pool / job-queue setup
+ *                       (nb_init.cpp:189-235, nb_main.cpp:55-98)
+ *   nomad_upload_train  the per-thread CSC of Data (nomad_body.h:34-46) built by
+ *   nomad_upload_test   read_data (nb_data.cpp:19-176); here one global CSC
+ *   nomad_set_factors   W block init (nb_main.cpp:444-456) and H token values
+ *   nomad_get_factors   (nb_main.cpp:70-98) / the --output dump (nb_main.cpp:
+ *                       350-366, 683-705)
+ *   nomad_train         one timeout segment of the token loop
+ *                       (nb_main.cpp:109-134 + 480-575)
+ *   nomad_eval          the test/train SSE pass (nb_main.cpp:601-676, 228-241)
+ *   nomad_destroy       teardown (nb_main.cpp:368-408)
+ *
+ * Conventions: every function returns 0 on success and a negative NOMAD_E*
+ * code on failure; nomad_last_error() then returns a message for the calling
+ * host thread.  No C++ exception crosses the boundary.  Calls are blocking
+ * with respect to the host.  One context is driven by one host thread.
+ */
+#ifndef NOMAD_HIP_H_
+#define NOMAD_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NOMAD_ABI_VERSION 1
+
+/* storage type of the factor matrices W (m x k) and H (n x k) */
+#define NOMAD_STORE_FP32 0 /* nomad_float  (Code/nomad/nomad_body.hpp:326-353) */
+#define NOMAD_STORE_BF16 1 /* bf16 storage, fp32 arithmetic                       */
+#define NOMAD_STORE_FP64 2 /* nomad_double (Code/nomad/nb_main.cpp:444,517)      */
+
+/* schedule */
+#define NOMAD_MODE_ASYNC 0 /* token passing through device ring queues        */
+#define NOMAD_MODE_STRAT 1 /* deterministic B x B stratified schedule          */
+
+/* rounding of bf16 stores */
+#define NOMAD_ROUND_NEAREST 0
+#define NOMAD_ROUND_STOCHASTIC 1
+
+/* error codes */
+#define NOMAD_OK 0
+#define NOMAD_EINVAL -1   /* bad argument                           */
+#define NOMAD_EHIP -2     /* HIP runtime error                      */
+#define NOMAD_ESTATE -3   /* call out of order (e.g. no data)       */
+#define NOMAD_ENOMEM -4   /* device allocation failed               */
+#define NOMAD_ETIMEOUT -5 /* device watchdog stopped a kernel       */
+
+typedef struct nomad_ctx nomad_ctx;
+
+typedef struct nomad_params {
+    int32_t m, n, k;   /* rows (users), columns (items), latent dimension (--dim) */
+    int32_t storage;   /* NOMAD_STORE_*                                          */
+    int32_t mode;      /* NOMAD_MODE_*                                           */
+    int32_t rounding;  /* NOMAD_ROUND_* (bf16 only)                              */
+    int32_t nworkers;  /* async: workers = row blocks (0: fill the GPU);
+                          strat: strata B (0: one per co-resident wave)          */
+    int32_t ptoken;    /* tokens a worker may hold ahead of the one it is
+                          processing (README --ptoken); 0 = default            */
+    int32_t device;    /* HIP device ordinal                                     */
+    int32_t flags;     /* reserved, must be 0                                    */
+    double lrate;      /* --lrate  */
+    double drate;      /* --drate  */
+    double reg;        /* --reg    */
+    uint64_t seed;     /* --seed: route permutations use mt19937_64(seed + 137)
+                          and re-permutations mt19937_64(seed + 139), as the
+                          reference does at one rank (nb_main.cpp:79, nb_init.cpp:219) */
+} nomad_params;
+
+typedef struct nomad_train_stats {
+    int64_t updates;      /* rating updates in this call (u=, nb_main.cpp:284)       */
+    int64_t visits;       /* non-empty (column, worker) visits in this call          */
+    int64_t failed_pops;  /* empty polls of a worker queue (f=, nb_main.cpp:284)     */
+    int64_t epochs_done;  /* completed passes of every column after the call         */
+    double train_seconds; /* device time of the training kernels                     */
+    double alg_bytes;     /* algorithmic HBM bytes of the call: nnz-updates *
+                             (4 + s_r + 2 k s) + visits * 2 k s                       */
+} nomad_train_stats;
+
+typedef struct nomad_eval_stats {
+    double train_sse;
+    int64_t train_count;
+    double test_sse;
+    int64_t test_count;
+    double seconds;
+} nomad_eval_stats;
+
+/* ---- context lifecycle ---- */
+int nomad_create(const nomad_params* params, nomad_ctx** out);
+int nomad_destroy(nomad_ctx* ctx);
+const char* nomad_last_error(void);
+int nomad_abi_version(void);
+
+/* Global CSC of the training / test ratings: colptr[n+1], rowidx[nnz] with
+ * rows ascending inside each column, val[nnz].  Host buffers; the library
+ * copies them.  Must be called before nomad_train / nomad_eval. */
+int nomad_upload_train(nomad_ctx* ctx, const int64_t* colptr, const int32_t* rowidx,
+                       const double* val, int64_t nnz);
+int nomad_upload_test(nomad_ctx* ctx, const int64_t* colptr, const int32_t* rowidx,
+                      const double* val, int64_t nnz);
+
+/* Row-major host factors, W m x k and H n x k, in double; converted to the
+ * storage type on upload (bf16: round to nearest even) and back on download.
+ * Either pointer may be NULL to skip that matrix. */
+int nomad_set_factors(nomad_ctx* ctx, const double* W, const double* H);
+int nomad_get_factors(nomad_ctx* ctx, double* W, double* H);
+
+/* One training segment.  Async mode runs until every column has completed
+ * `epochs` more passes over all row blocks (epochs > 0) or until `seconds` of
+ * device time have elapsed (seconds > 0), whichever comes first; at least one
+ * of the two must be positive.  Strat mode runs `epochs` full B x B sweeps. */
+int nomad_train(nomad_ctx* ctx, double seconds, int64_t epochs, nomad_train_stats* stats);
+
+/* Sum of squared errors over the training and test ratings, accumulated in
+ * FP64 (the dot product runs in the storage's arithmetic type). */
+int nomad_eval(nomad_ctx* ctx, nomad_eval_stats* stats);
+
+/* Number of workers / strata the context actually uses (after auto sizing). */
+int nomad_get_nworkers(nomad_ctx* ctx, int32_t* nworkers);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NOMAD_HIP_H_ */

@@ -1,0 +1,795 @@
+// Host side of the C ABI declared in include/nomad_hip.h.
+//
+// Owns the device copies of the ratings and factors, builds the column routes
+// (the reference's per-token thread permutation, Code/nomad/pool.cpp:32-38,
+// nb_main.cpp:79-97 and 313-321), seeds the device rings and launches the
+// kernels of sgd_kernels.hip / eval_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "kernel_args.h"
+#include "nomad_hip.h"
+
+using nomad_k::SgdArgs;
+using nomad_k::SseArgs;
+using nomad_k::VisitDesc;
+
+template <int S>
+__global__ void sgd_strat_kernel(SgdArgs);
+template <int S>
+__global__ void sgd_async_kernel(SgdArgs);
+template <int S>
+__global__ void sse_kernel(SseArgs);
+__global__ void seed_rings_kernel(SgdArgs, const int32_t*, const int64_t*);
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define NOMAD_HIP_TRY(expr)                                                                   \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(NOMAD_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+size_t elem_bytes(int storage) {
+    return storage == NOMAD_STORE_FP64 ? 8 : storage == NOMAD_STORE_BF16 ? 2 : 4;
+}
+
+uint16_t bf16_from_float_rne(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+
+float float_from_bf16(uint16_t b) {
+    uint32_t u = uint32_t(b) << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+template <class T>
+int dev_alloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+    if (e != hipSuccess)
+        return fail(NOMAD_ENOMEM, std::string("hipMalloc(") + std::to_string(count * sizeof(T)) +
+                                      " bytes): " + hipGetErrorString(e));
+    return NOMAD_OK;
+}
+
+template <class T>
+void dev_free(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+struct DevCsc {
+    int64_t nnz = -1;  // -1 = not uploaded
+    int64_t* colptr = nullptr;
+    int32_t* rowidx = nullptr;
+    void* val = nullptr;
+};
+
+}  // namespace
+
+struct nomad_ctx {
+    nomad_params p{};
+    int P = 0;  // workers (async) or strata (strat)
+    int rpb = 0, cpb = 0, ldf = 0;
+    size_t esz = 4, rvsz = 4;
+    uint32_t sr_seed = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double wall_hz = 100e6;
+
+    void* dW = nullptr;
+    void* dH = nullptr;
+    DevCsc tr, te;
+    std::vector<int64_t> h_colptr;  // training CSC kept on the host for routes
+    std::vector<int32_t> h_rowidx;
+
+    void* d_steps = nullptr;
+    int nsteps = 0;
+    double* d_partial = nullptr;
+    int64_t partial_cap = 0;
+
+    // async routes and rings
+    bool routes_ready = false;
+    int64_t V = 0;
+    std::vector<int64_t> h_vptr;
+    std::vector<VisitDesc> h_visits;
+    int64_t* d_vptr = nullptr;
+    VisitDesc* d_visits = nullptr;
+    int32_t* d_vcnt = nullptr;
+    int32_t* d_tok_pos = nullptr;
+    int32_t* d_tok_cycle = nullptr;
+    uint64_t* d_qslot = nullptr;
+    uint32_t* d_qtail = nullptr;
+    uint32_t* d_qhead = nullptr;
+    int qlog2 = 0;
+    uint32_t* d_done = nullptr;
+    uint32_t* d_timed_out = nullptr;
+    int64_t* d_stats = nullptr;
+    std::vector<int64_t> stats_prev;
+    int32_t ncols_active = 0;
+    std::mt19937_64 rng_route;   // initial permutations (nb_main.cpp:79)
+    std::mt19937_64 rng_reperm;  // re-permutation after a segment (nb_init.cpp:219, nb_main.cpp:316)
+    int segments = 0;
+    int64_t cycle_base = 0;      // route cycles completed by every active column
+    int64_t strat_epochs = 0;
+
+    ~nomad_ctx() {
+        dev_free(dW);
+        dev_free(dH);
+        for (DevCsc* c : {&tr, &te}) {
+            dev_free(c->colptr);
+            dev_free(c->rowidx);
+            dev_free(c->val);
+        }
+        dev_free(d_steps);
+        dev_free(d_partial);
+        dev_free(d_vptr);
+        dev_free(d_visits);
+        dev_free(d_vcnt);
+        dev_free(d_tok_pos);
+        dev_free(d_tok_cycle);
+        dev_free(d_qslot);
+        dev_free(d_qtail);
+        dev_free(d_qhead);
+        dev_free(d_done);
+        dev_free(d_timed_out);
+        dev_free(d_stats);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// Step table indexed by the visit count c (Code/nomad/nb_main.cpp:514-515):
+//   step(c) = lrate * 1.5 / (1 + drate * (c + 1)^1.5)
+// computed in double with the host libm; for float storage lrate and drate are
+// first rounded to float and the step is rounded to float, as in the float
+// build of the legacy updater (Code/nomad/nomad_body.hpp:313-320).
+int ensure_steps(nomad_ctx* c, int64_t need) {
+    if (need <= c->nsteps) return NOMAD_OK;
+    int n = int(std::max<int64_t>(need + 1024, 4096));
+    const bool is64 = c->p.storage == NOMAD_STORE_FP64;
+    std::vector<char> buf(size_t(n) * (is64 ? 8 : 4));
+    for (int i = 0; i < n; ++i) {
+        if (is64) {
+            double s = c->p.lrate * 1.5 / (1.0 + c->p.drate * std::pow(double(i) + 1.0, 1.5));
+            std::memcpy(buf.data() + size_t(i) * 8, &s, 8);
+        } else {
+            const float lr = float(c->p.lrate), dr = float(c->p.drate);
+            float s = float(double(lr) * 1.5 / (1.0 + double(dr) * std::pow(double(i) + 1.0, 1.5)));
+            std::memcpy(buf.data() + size_t(i) * 4, &s, 4);
+        }
+    }
+    dev_free(c->d_steps);
+    int rc = dev_alloc(reinterpret_cast<char**>(&c->d_steps), buf.size());
+    if (rc) return rc;
+    NOMAD_HIP_TRY(hipMemcpy(c->d_steps, buf.data(), buf.size(), hipMemcpyHostToDevice));
+    c->nsteps = n;
+    return NOMAD_OK;
+}
+
+int upload_csc(nomad_ctx* c, DevCsc& d, const int64_t* colptr, const int32_t* rowidx,
+               const double* val, int64_t nnz, bool keep_host) {
+    const int32_t m = c->p.m, n = c->p.n;
+    if (!colptr || (nnz > 0 && (!rowidx || !val)) || nnz < 0)
+        return fail(NOMAD_EINVAL, "upload: null pointer or negative nnz");
+    if (colptr[0] != 0 || colptr[n] != nnz) return fail(NOMAD_EINVAL, "upload: colptr does not span nnz");
+    for (int32_t j = 0; j < n; ++j) {
+        if (colptr[j + 1] < colptr[j]) return fail(NOMAD_EINVAL, "upload: colptr not monotone");
+        for (int64_t e = colptr[j]; e < colptr[j + 1]; ++e) {
+            if (rowidx[e] < 0 || rowidx[e] >= m) return fail(NOMAD_EINVAL, "upload: row index out of range");
+            if (e > colptr[j] && rowidx[e] < rowidx[e - 1])
+                return fail(NOMAD_EINVAL, "upload: rows not ascending within a column");
+        }
+    }
+    dev_free(d.colptr);
+    dev_free(d.rowidx);
+    dev_free(d.val);
+    int rc;
+    if ((rc = dev_alloc(&d.colptr, size_t(n) + 1))) return rc;
+    if ((rc = dev_alloc(&d.rowidx, size_t(nnz)))) return rc;
+    if ((rc = dev_alloc(reinterpret_cast<char**>(&d.val), size_t(nnz) * c->rvsz))) return rc;
+    NOMAD_HIP_TRY(hipMemcpy(d.colptr, colptr, (size_t(n) + 1) * 8, hipMemcpyHostToDevice));
+    if (nnz > 0) {
+        NOMAD_HIP_TRY(hipMemcpy(d.rowidx, rowidx, size_t(nnz) * 4, hipMemcpyHostToDevice));
+        if (c->rvsz == 8) {
+            NOMAD_HIP_TRY(hipMemcpy(d.val, val, size_t(nnz) * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> f(val, val + nnz);
+            NOMAD_HIP_TRY(hipMemcpy(d.val, f.data(), size_t(nnz) * 4, hipMemcpyHostToDevice));
+        }
+    }
+    d.nnz = nnz;
+    if (keep_host) {
+        c->h_colptr.assign(colptr, colptr + n + 1);
+        c->h_rowidx.assign(rowidx, rowidx + nnz);
+        c->routes_ready = false;
+    }
+    return NOMAD_OK;
+}
+
+// Visit lists: for column j, the non-empty (column, worker) pairs ordered by
+// the column's worker permutation.  perm is a std::shuffle of 0..P-1 drawn
+// from rng, one per column in column order, like ColumnData::set_perm
+// (Code/nomad/pool.cpp:32-38) with P workers in place of the reference's
+// threads.  Workers with no rating of the column are skipped: their visit in
+// the reference is an empty loop.
+void build_routes(nomad_ctx* c, std::mt19937_64& rng, const std::vector<int32_t>* old_counts,
+                  std::vector<int32_t>& counts_out) {
+    const int32_t n = c->p.n, P = c->P;
+    std::vector<int32_t> perm(P), rank(P);
+    std::vector<VisitDesc> runs;
+    std::vector<int32_t> run_cnt;
+    const bool first = c->h_vptr.empty();
+    std::vector<int64_t> vptr(size_t(n) + 1, 0);
+    std::vector<VisitDesc> visits;
+    visits.reserve(c->h_visits.size());
+    counts_out.clear();
+    counts_out.reserve(c->h_visits.size());
+    for (int32_t j = 0; j < n; ++j) {
+        for (int32_t q = 0; q < P; ++q) perm[q] = q;
+        std::shuffle(perm.begin(), perm.end(), rng);
+        for (int32_t q = 0; q < P; ++q) rank[perm[q]] = q;
+        runs.clear();
+        run_cnt.clear();
+        if (first) {
+            const int64_t c0 = c->h_colptr[j], c1 = c->h_colptr[j + 1];
+            for (int64_t e = c0; e < c1;) {
+                const int32_t wk = c->h_rowidx[e] / c->rpb;
+                int64_t f = e;
+                while (f < c1 && c->h_rowidx[f] / c->rpb == wk) ++f;
+                runs.push_back(VisitDesc{e, int32_t(f - e), wk});
+                run_cnt.push_back(0);
+                e = f;
+            }
+        } else {
+            for (int64_t v = c->h_vptr[j]; v < c->h_vptr[j + 1]; ++v) {
+                runs.push_back(c->h_visits[v]);
+                run_cnt.push_back((*old_counts)[v]);
+            }
+        }
+        std::vector<int32_t> order(runs.size());
+        std::iota(order.begin(), order.end(), 0);
+        std::sort(order.begin(), order.end(),
+                  [&](int32_t a, int32_t b) { return rank[runs[a].worker] < rank[runs[b].worker]; });
+        for (int32_t o : order) {
+            visits.push_back(runs[o]);
+            counts_out.push_back(run_cnt[o]);
+        }
+        vptr[j + 1] = int64_t(visits.size());
+    }
+    c->h_vptr.swap(vptr);
+    c->h_visits.swap(visits);
+    c->V = int64_t(c->h_visits.size());
+}
+
+int setup_async(nomad_ctx* c) {
+    std::vector<int32_t> counts;
+    int rc;
+    if (!c->routes_ready) {
+        c->h_vptr.clear();
+        c->h_visits.clear();
+        build_routes(c, c->rng_route, nullptr, counts);
+        const int32_t n = c->p.n, P = c->P;
+        // ring capacity: at least the number of columns that can ever be queued
+        // at one worker (its distinct columns) + 1
+        std::vector<int32_t> per_worker(P, 0);
+        c->ncols_active = 0;
+        for (int32_t j = 0; j < n; ++j) {
+            if (c->h_vptr[j + 1] > c->h_vptr[j]) ++c->ncols_active;
+            for (int64_t v = c->h_vptr[j]; v < c->h_vptr[j + 1]; ++v) per_worker[c->h_visits[v].worker]++;
+        }
+        const int32_t maxq = P ? *std::max_element(per_worker.begin(), per_worker.end()) : 0;
+        c->qlog2 = 1;
+        while ((int64_t(1) << c->qlog2) < int64_t(maxq) + 1) ++c->qlog2;
+        dev_free(c->d_vptr);
+        dev_free(c->d_visits);
+        dev_free(c->d_vcnt);
+        dev_free(c->d_tok_pos);
+        dev_free(c->d_tok_cycle);
+        dev_free(c->d_qslot);
+        dev_free(c->d_qtail);
+        dev_free(c->d_qhead);
+        dev_free(c->d_done);
+        dev_free(c->d_timed_out);
+        dev_free(c->d_stats);
+        if ((rc = dev_alloc(&c->d_vptr, size_t(n) + 1))) return rc;
+        if ((rc = dev_alloc(&c->d_visits, size_t(c->V)))) return rc;
+        if ((rc = dev_alloc(&c->d_vcnt, size_t(c->V)))) return rc;
+        if ((rc = dev_alloc(&c->d_tok_pos, size_t(n)))) return rc;
+        if ((rc = dev_alloc(&c->d_tok_cycle, size_t(n)))) return rc;
+        if ((rc = dev_alloc(&c->d_qslot, size_t(P) << c->qlog2))) return rc;
+        if ((rc = dev_alloc(&c->d_qtail, size_t(P)))) return rc;
+        if ((rc = dev_alloc(&c->d_qhead, size_t(P)))) return rc;
+        if ((rc = dev_alloc(&c->d_done, 1))) return rc;
+        if ((rc = dev_alloc(&c->d_timed_out, 1))) return rc;
+        if ((rc = dev_alloc(&c->d_stats, size_t(P) * 3))) return rc;
+        NOMAD_HIP_TRY(hipMemset(c->d_tok_cycle, 0, size_t(n) * 4));
+        NOMAD_HIP_TRY(hipMemset(c->d_qslot, 0, (size_t(P) << c->qlog2) * 8));
+        NOMAD_HIP_TRY(hipMemset(c->d_qtail, 0, size_t(P) * 4));
+        NOMAD_HIP_TRY(hipMemset(c->d_qhead, 0, size_t(P) * 4));
+        NOMAD_HIP_TRY(hipMemset(c->d_stats, 0, size_t(P) * 3 * 8));
+        c->stats_prev.assign(size_t(P) * 3, 0);
+        c->routes_ready = true;
+        c->segments = 0;
+        c->cycle_base = 0;
+    } else {
+        // Re-permute every column before a new segment, as the reference does
+        // after each evaluation (nb_main.cpp:313-321); visit counts move with
+        // their (column, worker) entry.
+        std::vector<int32_t> old(static_cast<size_t>(c->V));
+        NOMAD_HIP_TRY(hipMemcpy(old.data(), c->d_vcnt, size_t(c->V) * 4, hipMemcpyDeviceToHost));
+        build_routes(c, c->rng_reperm, &old, counts);
+        // drop whatever is still queued: head = tail on every ring
+        NOMAD_HIP_TRY(hipMemcpy(c->d_qhead, c->d_qtail, size_t(c->P) * 4, hipMemcpyDeviceToDevice));
+    }
+    NOMAD_HIP_TRY(hipMemcpy(c->d_vptr, c->h_vptr.data(), c->h_vptr.size() * 8, hipMemcpyHostToDevice));
+    NOMAD_HIP_TRY(hipMemcpy(c->d_visits, c->h_visits.data(), size_t(c->V) * sizeof(VisitDesc),
+                            hipMemcpyHostToDevice));
+    NOMAD_HIP_TRY(hipMemcpy(c->d_vcnt, counts.data(), size_t(c->V) * 4, hipMemcpyHostToDevice));
+    NOMAD_HIP_TRY(hipMemset(c->d_tok_pos, 0, size_t(c->p.n) * 4));
+    return NOMAD_OK;
+}
+
+SgdArgs base_args(nomad_ctx* c) {
+    SgdArgs a{};
+    a.m = c->p.m;
+    a.n = c->p.n;
+    a.k = c->p.k;
+    a.ldf = c->ldf;
+    a.nblocks = c->P;
+    a.rpb = c->rpb;
+    a.cpb = c->cpb;
+    a.round_mode = c->p.rounding;
+    a.sr_seed = c->sr_seed;
+    a.nsteps = c->nsteps;
+    a.W = c->dW;
+    a.H = c->dH;
+    a.colptr = c->tr.colptr;
+    a.rowidx = c->tr.rowidx;
+    a.val = c->tr.val;
+    a.steps = c->d_steps;
+    a.lrate = c->p.storage == NOMAD_STORE_FP64 ? c->p.lrate : double(float(c->p.lrate));
+    a.drate = c->p.storage == NOMAD_STORE_FP64 ? c->p.drate : double(float(c->p.drate));
+    a.reg = c->p.storage == NOMAD_STORE_FP64 ? c->p.reg : double(float(c->p.reg));
+    return a;
+}
+
+template <int S>
+hipError_t launch_strat(const SgdArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(sgd_strat_kernel<S>, dim3(a.nblocks), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int S>
+hipError_t launch_async(const SgdArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(sgd_async_kernel<S>, dim3(a.nblocks), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int S>
+hipError_t launch_sse(const SseArgs& a, int64_t waves, hipStream_t st) {
+    const int64_t blocks = (waves + 3) / 4;
+    hipLaunchKernelGGL(sse_kernel<S>, dim3(unsigned(blocks)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+template <class F>
+hipError_t dispatch_storage(int storage, F&& f) {
+    switch (storage) {
+        case NOMAD_STORE_FP32: return f(std::integral_constant<int, NOMAD_STORE_FP32>{});
+        case NOMAD_STORE_BF16: return f(std::integral_constant<int, NOMAD_STORE_BF16>{});
+        default: return f(std::integral_constant<int, NOMAD_STORE_FP64>{});
+    }
+}
+
+double watchdog_seconds() {
+    const char* s = std::getenv("NOMAD_WATCHDOG_SECONDS");
+    double v = s ? std::atof(s) : 0.0;
+    return v > 0 ? v : 120.0;
+}
+
+int train_async(nomad_ctx* c, double seconds, int64_t epochs, nomad_train_stats* st) {
+    int rc = setup_async(c);
+    if (rc) return rc;
+    const int32_t n = c->p.n, P = c->P;
+    // seed every active column at the first worker of its (new) route, in
+    // column order
+    std::vector<int64_t> off(size_t(P) + 1, 0);
+    for (int32_t j = 0; j < n; ++j)
+        if (c->h_vptr[j + 1] > c->h_vptr[j]) off[c->h_visits[c->h_vptr[j]].worker + 1]++;
+    for (int32_t q = 0; q < P; ++q) off[q + 1] += off[q];
+    std::vector<int32_t> list(static_cast<size_t>(off[P]));
+    {
+        std::vector<int64_t> fill(off.begin(), off.end() - 1);
+        for (int32_t j = 0; j < n; ++j)
+            if (c->h_vptr[j + 1] > c->h_vptr[j]) list[fill[c->h_visits[c->h_vptr[j]].worker]++] = j;
+    }
+    int32_t* d_list = nullptr;
+    int64_t* d_off = nullptr;
+    if ((rc = dev_alloc(&d_list, list.size()))) return rc;
+    if ((rc = dev_alloc(&d_off, off.size()))) {
+        dev_free(d_list);
+        return rc;
+    }
+    auto cleanup = [&] {
+        dev_free(d_list);
+        dev_free(d_off);
+    };
+    if (hipMemcpy(d_list, list.data(), list.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_off, off.data(), off.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+        cleanup();
+        return fail(NOMAD_EHIP, "copy of ring seeds failed");
+    }
+    const int64_t target = epochs > 0 ? c->cycle_base + epochs : int64_t(INT_MAX);
+    if ((rc = ensure_steps(c, std::min<int64_t>(target, c->cycle_base + 100000) + 2))) {
+        cleanup();
+        return rc;
+    }
+    SgdArgs a = base_args(c);
+    a.vptr = c->d_vptr;
+    a.visits = c->d_visits;
+    a.vcnt = c->d_vcnt;
+    a.tok_pos = c->d_tok_pos;
+    a.tok_cycle = c->d_tok_cycle;
+    a.qslot = c->d_qslot;
+    a.qtail = c->d_qtail;
+    a.qhead = c->d_qhead;
+    a.done = c->d_done;
+    a.stats = c->d_stats;
+    a.timed_out = c->d_timed_out;
+    a.qlog2 = c->qlog2;
+    a.target_cycle = int32_t(std::min<int64_t>(target, INT_MAX));
+    a.ncols_active = c->ncols_active;
+    const double limit = seconds > 0 ? seconds : watchdog_seconds();
+    a.deadline = uint64_t(limit * c->wall_hz);
+    hipError_t e = hipMemsetAsync(c->d_done, 0, 4, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->d_timed_out, 0, 4, c->stream);
+    if (e == hipSuccess) {
+        const int threads = 256;
+        hipLaunchKernelGGL(seed_rings_kernel, dim3((P + threads - 1) / threads), dim3(threads), 0,
+                           c->stream, a, d_list, d_off);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+    if (e == hipSuccess)
+        e = dispatch_storage(c->p.storage, [&](auto s) { return launch_async<decltype(s)::value>(a, c->stream); });
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    cleanup();
+    if (e != hipSuccess) return fail(NOMAD_EHIP, std::string("async training launch: ") + hipGetErrorString(e));
+    float ms = 0.f;
+    NOMAD_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    uint32_t done = 0, timed_out = 0;
+    NOMAD_HIP_TRY(hipMemcpy(&done, c->d_done, 4, hipMemcpyDeviceToHost));
+    NOMAD_HIP_TRY(hipMemcpy(&timed_out, c->d_timed_out, 4, hipMemcpyDeviceToHost));
+    std::vector<int64_t> stats(size_t(P) * 3);
+    NOMAD_HIP_TRY(hipMemcpy(stats.data(), c->d_stats, stats.size() * 8, hipMemcpyDeviceToHost));
+    int64_t upd = 0, vis = 0, fails = 0;
+    for (int32_t q = 0; q < P; ++q) {
+        upd += stats[3 * q] - c->stats_prev[3 * q];
+        vis += stats[3 * q + 1] - c->stats_prev[3 * q + 1];
+        fails += stats[3 * q + 2] - c->stats_prev[3 * q + 2];
+    }
+    c->stats_prev.swap(stats);
+    ++c->segments;
+    const bool all_parked = int32_t(done) >= c->ncols_active;
+    if (all_parked && epochs > 0) {
+        c->cycle_base = target;
+    } else {
+        std::vector<int32_t> cyc(static_cast<size_t>(n));
+        NOMAD_HIP_TRY(hipMemcpy(cyc.data(), c->d_tok_cycle, size_t(n) * 4, hipMemcpyDeviceToHost));
+        int64_t mn = INT64_MAX;
+        for (int32_t j = 0; j < n; ++j)
+            if (c->h_vptr[j + 1] > c->h_vptr[j]) mn = std::min<int64_t>(mn, cyc[j]);
+        c->cycle_base = mn == INT64_MAX ? c->cycle_base : mn;
+    }
+    if (st) {
+        st->updates = upd;
+        st->visits = vis;
+        st->failed_pops = fails;
+        st->epochs_done = c->cycle_base;
+        st->train_seconds = ms * 1e-3;
+        const double ks = double(c->p.k) * double(c->esz);
+        st->alg_bytes = double(upd) * (4.0 + double(c->rvsz) + 2.0 * ks) + double(vis) * 2.0 * ks;
+    }
+    if (seconds <= 0 && !all_parked)
+        return fail(NOMAD_ETIMEOUT, "async training stopped by the device watchdog after " +
+                                        std::to_string(limit) + " s before finishing its epochs");
+    (void)timed_out;
+    return NOMAD_OK;
+}
+
+int train_strat(nomad_ctx* c, int64_t epochs, nomad_train_stats* st) {
+    if (epochs <= 0) return fail(NOMAD_EINVAL, "strat mode needs epochs > 0");
+    int rc = ensure_steps(c, c->strat_epochs + epochs + 2);
+    if (rc) return rc;
+    SgdArgs a = base_args(c);
+    NOMAD_HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    for (int64_t ep = 0; ep < epochs; ++ep) {
+        a.epoch = int32_t(c->strat_epochs + ep);
+        for (int s = 0; s < c->P; ++s) {
+            a.substep = s;
+            hipError_t e = dispatch_storage(c->p.storage,
+                                            [&](auto t) { return launch_strat<decltype(t)::value>(a, c->stream); });
+            if (e != hipSuccess) return fail(NOMAD_EHIP, std::string("strat launch: ") + hipGetErrorString(e));
+        }
+    }
+    NOMAD_HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    NOMAD_HIP_TRY(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    NOMAD_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->strat_epochs += epochs;
+    if (st) {
+        // every rating once per epoch; visits = non-empty (column, row block) pairs
+        int64_t vis = 0;
+        for (int32_t j = 0; j < c->p.n; ++j) {
+            int32_t last = -1;
+            for (int64_t e = c->h_colptr[j]; e < c->h_colptr[j + 1]; ++e) {
+                const int32_t b = c->h_rowidx[e] / c->rpb;
+                if (b != last) {
+                    ++vis;
+                    last = b;
+                }
+            }
+        }
+        st->updates = c->tr.nnz * epochs;
+        st->visits = vis * epochs;
+        st->failed_pops = 0;
+        st->epochs_done = c->strat_epochs;
+        st->train_seconds = ms * 1e-3;
+        const double ks = double(c->p.k) * double(c->esz);
+        st->alg_bytes = double(st->updates) * (4.0 + double(c->rvsz) + 2.0 * ks) + double(st->visits) * 2.0 * ks;
+    }
+    return NOMAD_OK;
+}
+
+int eval_one(nomad_ctx* c, const DevCsc& d, double* sse) {
+    *sse = 0.0;
+    if (d.nnz <= 0) return NOMAD_OK;
+    const int64_t chunk = 512;
+    const int64_t waves = (d.nnz + chunk - 1) / chunk;
+    if (waves > c->partial_cap) {
+        dev_free(c->d_partial);
+        int rc = dev_alloc(&c->d_partial, size_t(waves + 4));
+        if (rc) return rc;
+        c->partial_cap = waves + 4;
+    }
+    SseArgs a{};
+    a.m = c->p.m;
+    a.n = c->p.n;
+    a.k = c->p.k;
+    a.ldf = c->ldf;
+    a.W = c->dW;
+    a.H = c->dH;
+    a.colptr = d.colptr;
+    a.rowidx = d.rowidx;
+    a.val = d.val;
+    a.nnz = d.nnz;
+    a.chunk = chunk;
+    a.partial = c->d_partial;
+    hipError_t e = dispatch_storage(c->p.storage,
+                                    [&](auto t) { return launch_sse<decltype(t)::value>(a, waves, c->stream); });
+    if (e != hipSuccess) return fail(NOMAD_EHIP, std::string("sse launch: ") + hipGetErrorString(e));
+    std::vector<double> part(static_cast<size_t>(waves));
+    NOMAD_HIP_TRY(hipMemcpyAsync(part.data(), c->d_partial, size_t(waves) * 8, hipMemcpyDeviceToHost, c->stream));
+    NOMAD_HIP_TRY(hipStreamSynchronize(c->stream));
+    double s = 0.0;
+    for (double v : part) s += v;
+    *sse = s;
+    return NOMAD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* nomad_last_error(void) { return g_last_error.c_str(); }
+
+int nomad_abi_version(void) { return NOMAD_ABI_VERSION; }
+
+int nomad_create(const nomad_params* p, nomad_ctx** out) {
+    if (!p || !out) return fail(NOMAD_EINVAL, "nomad_create: null argument");
+    *out = nullptr;
+    if (p->m <= 0 || p->n <= 0 || p->k <= 0) return fail(NOMAD_EINVAL, "nomad_create: m, n, k must be positive");
+    if (p->k > 128) return fail(NOMAD_EINVAL, "nomad_create: k > 128 is not supported (one element pair per lane)");
+    if (p->storage < NOMAD_STORE_FP32 || p->storage > NOMAD_STORE_FP64)
+        return fail(NOMAD_EINVAL, "nomad_create: unknown storage type");
+    if (p->mode != NOMAD_MODE_ASYNC && p->mode != NOMAD_MODE_STRAT)
+        return fail(NOMAD_EINVAL, "nomad_create: unknown mode");
+    if (p->rounding != NOMAD_ROUND_NEAREST && p->rounding != NOMAD_ROUND_STOCHASTIC)
+        return fail(NOMAD_EINVAL, "nomad_create: unknown rounding");
+    if (p->nworkers < 0 || p->ptoken < 0 || p->flags != 0)
+        return fail(NOMAD_EINVAL, "nomad_create: negative nworkers/ptoken or nonzero flags");
+    if (!(p->lrate > 0) || !(p->drate >= 0) || !(p->reg >= 0))
+        return fail(NOMAD_EINVAL, "nomad_create: lrate must be > 0, drate and reg >= 0");
+    int ndev = 0;
+    NOMAD_HIP_TRY(hipGetDeviceCount(&ndev));
+    if (p->device < 0 || p->device >= ndev) return fail(NOMAD_EINVAL, "nomad_create: no such device");
+    NOMAD_HIP_TRY(hipSetDevice(p->device));
+
+    nomad_ctx* c = new (std::nothrow) nomad_ctx();
+    if (!c) return fail(NOMAD_ENOMEM, "nomad_create: out of host memory");
+    c->p = *p;
+    c->esz = elem_bytes(p->storage);
+    c->rvsz = p->storage == NOMAD_STORE_FP64 ? 8 : 4;
+    c->ldf = (p->k + 1) & ~1;
+    c->sr_seed = uint32_t(p->seed ^ (p->seed >> 32)) ^ 0xA5A5A5A5u;
+    c->rng_route.seed(p->seed + 137);
+    c->rng_reperm.seed(p->seed + 139);
+    int rate_khz = 0;
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, p->device) == hipSuccess && rate_khz > 0)
+        c->wall_hz = double(rate_khz) * 1e3;
+
+    auto bail = [&](int rc) {
+        delete c;
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+        return bail(fail(NOMAD_EHIP, "nomad_create: stream/event creation failed"));
+
+    int P = p->nworkers;
+    if (p->mode == NOMAD_MODE_ASYNC) {
+        if (P == 0) {
+            int cus = 0, per_cu = 0;
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sgd_async_kernel<NOMAD_STORE_FP32>, 64, 0) !=
+                    hipSuccess)
+                return bail(fail(NOMAD_EHIP, "nomad_create: occupancy query failed"));
+            P = cus * std::max(1, std::min(per_cu, 4));
+        }
+    } else if (P == 0) {
+        P = 64;
+    }
+    P = std::max(1, std::min(P, p->m));
+    c->P = P;
+    c->rpb = p->m / P + ((p->m % P) > 0 ? 1 : 0);  // nb_data.cpp:83
+    if (p->mode == NOMAD_MODE_STRAT) c->cpb = p->n / P + ((p->n % P) > 0 ? 1 : 0);
+
+    int rc;
+    if ((rc = dev_alloc(reinterpret_cast<char**>(&c->dW), size_t(p->m) * c->ldf * c->esz))) return bail(rc);
+    if ((rc = dev_alloc(reinterpret_cast<char**>(&c->dH), size_t(p->n) * c->ldf * c->esz))) return bail(rc);
+    if (hipMemset(c->dW, 0, size_t(p->m) * c->ldf * c->esz) != hipSuccess ||
+        hipMemset(c->dH, 0, size_t(p->n) * c->ldf * c->esz) != hipSuccess)
+        return bail(fail(NOMAD_EHIP, "nomad_create: memset failed"));
+    *out = c;
+    return NOMAD_OK;
+}
+
+int nomad_destroy(nomad_ctx* c) {
+    if (!c) return NOMAD_OK;
+    (void)hipSetDevice(c->p.device);
+    (void)hipStreamSynchronize(c->stream);
+    delete c;
+    return NOMAD_OK;
+}
+
+int nomad_get_nworkers(nomad_ctx* c, int32_t* nworkers) {
+    if (!c || !nworkers) return fail(NOMAD_EINVAL, "nomad_get_nworkers: null argument");
+    *nworkers = c->P;
+    return NOMAD_OK;
+}
+
+int nomad_upload_train(nomad_ctx* c, const int64_t* colptr, const int32_t* rowidx, const double* val,
+                       int64_t nnz) {
+    if (!c) return fail(NOMAD_EINVAL, "nomad_upload_train: null context");
+    NOMAD_HIP_TRY(hipSetDevice(c->p.device));
+    return upload_csc(c, c->tr, colptr, rowidx, val, nnz, true);
+}
+
+int nomad_upload_test(nomad_ctx* c, const int64_t* colptr, const int32_t* rowidx, const double* val,
+                      int64_t nnz) {
+    if (!c) return fail(NOMAD_EINVAL, "nomad_upload_test: null context");
+    NOMAD_HIP_TRY(hipSetDevice(c->p.device));
+    return upload_csc(c, c->te, colptr, rowidx, val, nnz, false);
+}
+
+int nomad_set_factors(nomad_ctx* c, const double* W, const double* H) {
+    if (!c) return fail(NOMAD_EINVAL, "nomad_set_factors: null context");
+    NOMAD_HIP_TRY(hipSetDevice(c->p.device));
+    const int k = c->p.k, ldf = c->ldf;
+    for (int which = 0; which < 2; ++which) {
+        const double* src = which == 0 ? W : H;
+        if (!src) continue;
+        const int64_t rows = which == 0 ? c->p.m : c->p.n;
+        std::vector<char> buf(size_t(rows) * ldf * c->esz, 0);
+        for (int64_t r = 0; r < rows; ++r) {
+            for (int t = 0; t < k; ++t) {
+                const double v = src[r * k + t];
+                const size_t o = size_t(r) * ldf + t;
+                if (c->p.storage == NOMAD_STORE_FP64) {
+                    reinterpret_cast<double*>(buf.data())[o] = v;
+                } else if (c->p.storage == NOMAD_STORE_FP32) {
+                    reinterpret_cast<float*>(buf.data())[o] = float(v);
+                } else {
+                    reinterpret_cast<uint16_t*>(buf.data())[o] = bf16_from_float_rne(float(v));
+                }
+            }
+        }
+        NOMAD_HIP_TRY(hipMemcpy(which == 0 ? c->dW : c->dH, buf.data(), buf.size(), hipMemcpyHostToDevice));
+    }
+    return NOMAD_OK;
+}
+
+int nomad_get_factors(nomad_ctx* c, double* W, double* H) {
+    if (!c) return fail(NOMAD_EINVAL, "nomad_get_factors: null context");
+    NOMAD_HIP_TRY(hipSetDevice(c->p.device));
+    NOMAD_HIP_TRY(hipStreamSynchronize(c->stream));
+    const int k = c->p.k, ldf = c->ldf;
+    for (int which = 0; which < 2; ++which) {
+        double* dst = which == 0 ? W : H;
+        if (!dst) continue;
+        const int64_t rows = which == 0 ? c->p.m : c->p.n;
+        std::vector<char> buf(size_t(rows) * ldf * c->esz);
+        NOMAD_HIP_TRY(hipMemcpy(buf.data(), which == 0 ? c->dW : c->dH, buf.size(), hipMemcpyDeviceToHost));
+        for (int64_t r = 0; r < rows; ++r) {
+            for (int t = 0; t < k; ++t) {
+                const size_t o = size_t(r) * ldf + t;
+                double v;
+                if (c->p.storage == NOMAD_STORE_FP64)
+                    v = reinterpret_cast<const double*>(buf.data())[o];
+                else if (c->p.storage == NOMAD_STORE_FP32)
+                    v = reinterpret_cast<const float*>(buf.data())[o];
+                else
+                    v = float_from_bf16(reinterpret_cast<const uint16_t*>(buf.data())[o]);
+                dst[r * k + t] = v;
+            }
+        }
+    }
+    return NOMAD_OK;
+}
+
+int nomad_train(nomad_ctx* c, double seconds, int64_t epochs, nomad_train_stats* st) {
+    if (!c) return fail(NOMAD_EINVAL, "nomad_train: null context");
+    if (c->tr.nnz < 0) return fail(NOMAD_ESTATE, "nomad_train: no training data uploaded");
+    if (!(seconds > 0) && epochs <= 0) return fail(NOMAD_EINVAL, "nomad_train: need seconds > 0 or epochs > 0");
+    NOMAD_HIP_TRY(hipSetDevice(c->p.device));
+    if (st) std::memset(st, 0, sizeof(*st));
+    if (c->p.mode == NOMAD_MODE_STRAT) return train_strat(c, epochs, st);
+    return train_async(c, seconds, epochs, st);
+}
+
+int nomad_eval(nomad_ctx* c, nomad_eval_stats* st) {
+    if (!c || !st) return fail(NOMAD_EINVAL, "nomad_eval: null argument");
+    NOMAD_HIP_TRY(hipSetDevice(c->p.device));
+    std::memset(st, 0, sizeof(*st));
+    auto t0 = std::chrono::steady_clock::now();
+    int rc;
+    if ((rc = eval_one(c, c->tr, &st->train_sse))) return rc;
+    if ((rc = eval_one(c, c->te, &st->test_sse))) return rc;
+    st->train_count = std::max<int64_t>(c->tr.nnz, 0);
+    st->test_count = std::max<int64_t>(c->te.nnz, 0);
+    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return NOMAD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,224 @@
+// Device-side building blocks shared by the SGD, eval and gradient kernels.
+//
+// Factor layout: a factor row of k <= 128 elements is spread over the 64 lanes
+// of a wavefront as element pairs, lane l holding elements (2l, 2l+1).  One
+// row is therefore one coalesced load per wave: 8 B per lane in FP32, 4 B per
+// lane in BF16 (two packed bf16), 16 B per lane in FP64.  Lanes past the row
+// hold zeros and never store.
+//
+// Wave dot product: each lane forms p = x0*y0 then p = fma(x1, y1, p); the 64
+// partials are summed by a fixed tree that is reproduced bit-for-bit by the
+// CPU oracle (oracle/nomad_oracle.cpp, gpu_dot):
+//   Q_q   = (p[4q] + p[4q+1]) + (p[4q+2] + p[4q+3])          (quad_perm DPP)
+//   R_r   = (Q_{4r+3} + Q_{4r+2}) + (Q_{4r+1} + Q_{4r})        (row_shr:4, :8)
+//   total = (R_3 + R_2) + (R_1 + R_0)                          (row_bcast:15, :31)
+// read back from lane 63.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nomad_hip.h"
+
+namespace nomad_dev {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// ---- DPP helpers (gfx9 encodings) ----
+// quad_perm [1,0,3,2] = 0xB1, quad_perm [2,3,0,1] = 0x4E, row_shr:4 = 0x114,
+// row_shr:8 = 0x118, row_bcast:15 = 0x142, row_bcast:31 = 0x143.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_f32(float x) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, ROW_MASK, 0xF, true));
+}
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double x) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(uint32_t(u)), CTRL, ROW_MASK, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(uint32_t(u >> 32)), CTRL, ROW_MASK, 0xF, true);
+    return __builtin_bit_cast(double, (uint64_t(uint32_t(hi)) << 32) | uint32_t(lo));
+}
+
+__device__ __forceinline__ float wave_tree_sum(float x) {
+    x = x + dpp_f32<0xB1, 0xF>(x);
+    x = x + dpp_f32<0x4E, 0xF>(x);
+    x = x + dpp_f32<0x114, 0xF>(x);
+    x = x + dpp_f32<0x118, 0xF>(x);
+    x = x + dpp_f32<0x142, 0xA>(x);
+    x = x + dpp_f32<0x143, 0xC>(x);
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), 63));
+}
+
+__device__ __forceinline__ double wave_tree_sum(double x) {
+    x = x + dpp_f64<0xB1, 0xF>(x);
+    x = x + dpp_f64<0x4E, 0xF>(x);
+    x = x + dpp_f64<0x114, 0xF>(x);
+    x = x + dpp_f64<0x118, 0xF>(x);
+    x = x + dpp_f64<0x142, 0xA>(x);
+    x = x + dpp_f64<0x143, 0xC>(x);
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = __builtin_amdgcn_readlane(int(uint32_t(u)), 63);
+    const uint32_t hi = __builtin_amdgcn_readlane(int(uint32_t(u >> 32)), 63);
+    return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+
+// ---- bf16 <-> f32 ----
+__device__ __forceinline__ float bf16_lo(uint32_t u) { return __uint_as_float(u << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t u) { return __uint_as_float(u & 0xFFFF0000u); }
+
+__device__ __forceinline__ uint32_t f32_to_bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+
+// Stochastic rounding: add 16 uniform random bits below the bf16 cut, truncate.
+__device__ __forceinline__ uint32_t f32_to_bf16_sr(float f, uint32_t r16) {
+    uint32_t u = __float_as_uint(f);
+    u += (r16 & 0xFFFFu);
+    return u >> 16;
+}
+
+// 32-bit avalanche mixer (lowbias32) used to derive stochastic-rounding bits
+// from (seed, row/column, update counter, lane).
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t sr_bits(uint32_t seed, uint32_t a, uint32_t b, uint32_t lane) {
+    return mix32(seed ^ mix32(a * 0x9E3779B1u + b) ^ (lane * 0x85EBCA6Bu));
+}
+
+// ---- storage traits: load/store one factor row as per-lane pairs ----
+// Rows have an even leading dimension ldf >= k; the padding element (k odd)
+// is stored as zero and stays zero under the update.  npairs = ldf / 2 lanes
+// carry data; the others read the all-zero block g_zero_pairs and never
+// store.  A load is an address select plus one memory instruction, with no
+// data-dependent select after it, so several rows can be in flight.
+__device__ __align__(16) static double g_zero_pairs[2 * kWave];  // zero-initialised, never written
+
+template <int STORE>
+struct Store;
+
+template <>
+struct Store<NOMAD_STORE_FP32> {
+    using elem = float;
+    using acc = float;
+    using rval = float;
+    struct pair {
+        float x, y;
+    };
+    static __device__ __forceinline__ pair load(const elem* row, int lane, int npairs) {
+        const float2* src = lane < npairs ? reinterpret_cast<const float2*>(row)
+                                          : reinterpret_cast<const float2*>(g_zero_pairs);
+        const float2 v = src[lane];
+        return pair{v.x, v.y};
+    }
+    static __device__ __forceinline__ void store(elem* row, int lane, int npairs, pair p, uint32_t,
+                                                 int) {
+        if (lane < npairs) reinterpret_cast<float2*>(row)[lane] = make_float2(p.x, p.y);
+    }
+};
+
+template <>
+struct Store<NOMAD_STORE_FP64> {
+    using elem = double;
+    using acc = double;
+    using rval = double;
+    struct pair {
+        double x, y;
+    };
+    static __device__ __forceinline__ pair load(const elem* row, int lane, int npairs) {
+        const double2* src = lane < npairs ? reinterpret_cast<const double2*>(row)
+                                           : reinterpret_cast<const double2*>(g_zero_pairs);
+        const double2 v = src[lane];
+        return pair{v.x, v.y};
+    }
+    static __device__ __forceinline__ void store(elem* row, int lane, int npairs, pair p, uint32_t,
+                                                 int) {
+        if (lane < npairs) reinterpret_cast<double2*>(row)[lane] = make_double2(p.x, p.y);
+    }
+};
+
+template <>
+struct Store<NOMAD_STORE_BF16> {
+    using elem = uint16_t;
+    using acc = float;
+    using rval = float;
+    struct pair {
+        float x, y;
+    };
+    // element pair (2l, 2l+1) is one aligned 32-bit word
+    static __device__ __forceinline__ pair load(const elem* row, int lane, int npairs) {
+        const uint32_t* src = lane < npairs ? reinterpret_cast<const uint32_t*>(row)
+                                            : reinterpret_cast<const uint32_t*>(g_zero_pairs);
+        const uint32_t u = src[lane];
+        return pair{bf16_lo(u), bf16_hi(u)};
+    }
+    static __device__ __forceinline__ void store(elem* row, int lane, int npairs, pair p, uint32_t rnd,
+                                                 int round_mode) {
+        uint32_t lo, hi;
+        if (round_mode == NOMAD_ROUND_STOCHASTIC) {
+            lo = f32_to_bf16_sr(p.x, rnd);
+            hi = f32_to_bf16_sr(p.y, rnd >> 16);
+        } else {
+            lo = f32_to_bf16_rne(p.x);
+            hi = f32_to_bf16_rne(p.y);
+        }
+        if (lane < npairs) reinterpret_cast<uint32_t*>(row)[lane] = lo | (hi << 16);
+    }
+};
+
+// One SGD step on (w_i, h_j) given the rating r.  Per lane, elements x and y:
+//   p   = h.x*w.x ; p = fma(h.y, w.y, p) ; dot = tree(p) ; err = dot - r
+//   a   = fma(err, h, reg*w)     b = fma(err, w, reg*h)
+//   w'  = fma(-step, a, w)       h' = fma(-step, b, h)
+// This is the reference update of Code/nomad/nb_main.cpp:528-538 (w and h
+// updated simultaneously from their old values) in fused form.
+template <class P, class A>
+__device__ __forceinline__ A sgd_step(P& w, P& h, A r, A step, A reg) {
+    A p = h.x * w.x;
+    p = __builtin_fma(h.y, w.y, p);
+    const A dot = wave_tree_sum(p);
+    const A err = dot - r;
+    const A ax = __builtin_fma(err, h.x, reg * w.x);
+    const A ay = __builtin_fma(err, h.y, reg * w.y);
+    const A bx = __builtin_fma(err, w.x, reg * h.x);
+    const A by = __builtin_fma(err, w.y, reg * h.y);
+    w.x = __builtin_fma(-step, ax, w.x);
+    w.y = __builtin_fma(-step, ay, w.y);
+    h.x = __builtin_fma(-step, bx, h.x);
+    h.y = __builtin_fma(-step, by, h.y);
+    return err;
+}
+
+template <class A>
+__device__ __forceinline__ A wave_dot(A hx, A hy, A wx, A wy) {
+    A p = hx * wx;
+    p = __builtin_fma(hy, wy, p);
+    return wave_tree_sum(p);
+}
+
+__device__ __forceinline__ int readlane_i32(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+__device__ __forceinline__ float readlane_f32(float x, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
+}
+__device__ __forceinline__ double readlane_f64(double x, int l) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = __builtin_amdgcn_readlane(int(uint32_t(u)), l);
+    const uint32_t hi = __builtin_amdgcn_readlane(int(uint32_t(u >> 32)), l);
+    return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+__device__ __forceinline__ float readlane_val(float x, int l) { return readlane_f32(x, l); }
+__device__ __forceinline__ double readlane_val(double x, int l) { return readlane_f64(x, l); }
+
+}  // namespace nomad_dev

@@ -1,0 +1,69 @@
+// Kernel argument blocks shared by the host side (capi.hip) and the kernels.
+#pragma once
+
+#include <stdint.h>
+
+namespace nomad_k {
+
+// One non-empty (column, worker) visit: the worker's ratings of the column are
+// the contiguous CSC range [start, start + len) (rows ascending).
+struct VisitDesc {
+    int64_t start;
+    int32_t len;
+    int32_t worker;
+};
+
+struct SgdArgs {
+    // problem
+    int32_t m, n, k, ldf;
+    int32_t nblocks;      // workers (async) or strata B (strat)
+    int32_t rpb;          // rows per row block
+    int32_t cpb;          // columns per column block (strat)
+    int32_t round_mode;   // NOMAD_ROUND_*
+    uint32_t sr_seed;
+    int32_t nsteps;       // entries in the step table
+    void* W;              // m x ldf, storage type
+    void* H;              // n x ldf, storage type
+    const int64_t* colptr;
+    const int32_t* rowidx;
+    const void* val;      // float (fp32/bf16 storage) or double (fp64)
+    const void* steps;    // step table indexed by visit count: float or double
+    double lrate, drate;  // for counts past the table
+    double reg;
+
+    // async schedule
+    const int64_t* vptr;       // n + 1 offsets into visits
+    const VisitDesc* visits;   // per column, in route order
+    int32_t* vcnt;             // visit counts (step-size counter) per visit entry
+    int32_t* tok_pos;          // per column: index of its next visit in its route
+    int32_t* tok_cycle;        // per column: completed route cycles
+    uint64_t* qslot;           // nblocks x (1 << qlog2) ring slots: (lap+1) << 32 | column
+    uint32_t* qtail;           // per worker, producers' reservation counter
+    uint32_t* qhead;           // per worker, consumer position (persisted across launches)
+    uint32_t* done;            // columns parked after reaching target_cycle
+    int64_t* stats;            // per worker: updates, visits, failed pops
+    uint32_t* timed_out;       // set when a worker stopped on the deadline
+    int32_t qlog2;
+    int32_t target_cycle;      // park a column when tok_cycle reaches this
+    int32_t ncols_active;      // columns circulating in this launch
+    int32_t pad0;
+    uint64_t deadline;         // wall_clock64 ticks after its start at which a worker stops
+
+    // strat schedule
+    int32_t substep;
+    int32_t epoch;
+};
+
+struct SseArgs {
+    int32_t m, n, k, ldf;
+    const void* W;
+    const void* H;
+    const int64_t* colptr;
+    const int32_t* rowidx;
+    const void* val;
+    int64_t nnz;
+    int64_t chunk;         // ratings per wave
+    double* partial;       // one FP64 partial per wave
+};
+
+}  // namespace nomad_k

@@ -1,0 +1,316 @@
+"""ctypes bindings of libnomad_hip.so (include/nomad_hip.h) and
+libnomad_tools.so (include/nomad_tools.h).
+
+The Python layer is plumbing: every computation runs in the native libraries.
+If libnomad_hip.so is missing the import of `hip()` raises; there is no
+fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+LIBDIR = Path(__file__).resolve().parent / "lib"
+
+STORE_FP32, STORE_BF16, STORE_FP64 = 0, 1, 2
+MODE_ASYNC, MODE_STRAT = 0, 1
+ROUND_NEAREST, ROUND_STOCHASTIC = 0, 1
+STORAGE_NAMES = {"fp32": STORE_FP32, "bf16": STORE_BF16, "fp64": STORE_FP64}
+
+
+class NomadError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"nomad error {code}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("m", C.c_int32), ("n", C.c_int32), ("k", C.c_int32),
+        ("storage", C.c_int32), ("mode", C.c_int32), ("rounding", C.c_int32),
+        ("nworkers", C.c_int32), ("ptoken", C.c_int32), ("device", C.c_int32), ("flags", C.c_int32),
+        ("lrate", C.c_double), ("drate", C.c_double), ("reg", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
+class TrainStats(C.Structure):
+    _fields_ = [
+        ("updates", C.c_int64), ("visits", C.c_int64), ("failed_pops", C.c_int64),
+        ("epochs_done", C.c_int64), ("train_seconds", C.c_double), ("alg_bytes", C.c_double),
+    ]
+
+
+class EvalStats(C.Structure):
+    _fields_ = [
+        ("train_sse", C.c_double), ("train_count", C.c_int64),
+        ("test_sse", C.c_double), ("test_count", C.c_int64), ("seconds", C.c_double),
+    ]
+
+
+class SynthSpec(C.Structure):
+    _fields_ = [
+        ("m", C.c_int32), ("n", C.c_int32), ("nnz_train", C.c_int64), ("nnz_test", C.c_int64),
+        ("rank", C.c_int32), ("degree", C.c_int32), ("rating", C.c_int32), ("threads", C.c_int32),
+        ("factor_var", C.c_double), ("noise_std", C.c_double), ("row_exp", C.c_double),
+        ("col_exp", C.c_double), ("center", C.c_double), ("scale", C.c_double),
+        ("lo", C.c_double), ("hi", C.c_double), ("seed", C.c_uint64),
+    ]
+
+
+_P = C.c_void_p
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+
+# symbol -> (restype, argtypes); the header declarations, one for one
+HIP_SYMBOLS = {
+    "nomad_create": (C.c_int, [C.POINTER(Params), C.POINTER(_P)]),
+    "nomad_destroy": (C.c_int, [_P]),
+    "nomad_last_error": (C.c_char_p, []),
+    "nomad_abi_version": (C.c_int, []),
+    "nomad_upload_train": (C.c_int, [_P, _i64p, _i32p, _f64p, C.c_int64]),
+    "nomad_upload_test": (C.c_int, [_P, _i64p, _i32p, _f64p, C.c_int64]),
+    "nomad_set_factors": (C.c_int, [_P, _f64p, _f64p]),
+    "nomad_get_factors": (C.c_int, [_P, _f64p, _f64p]),
+    "nomad_train": (C.c_int, [_P, C.c_double, C.c_int64, C.POINTER(TrainStats)]),
+    "nomad_eval": (C.c_int, [_P, C.POINTER(EvalStats)]),
+    "nomad_get_nworkers": (C.c_int, [_P, _i32p]),
+}
+
+TOOLS_SYMBOLS = {
+    "nomad_tools_last_error": (C.c_char_p, []),
+    "nomad_tools_read_dat": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "nomad_tools_read_dat_header": (C.c_int, [C.c_char_p, _i32p, _i32p, _i64p]),
+    "nomad_tools_write_dat": (C.c_int, [C.c_char_p, _P, C.c_int32]),
+    "nomad_tools_matrix_from_csc": (C.c_int, [C.c_int32, C.c_int32, _i64p, _i32p, _f64p, C.POINTER(_P)]),
+    "nomad_tools_matrix_info": (C.c_int, [_P, _i32p, _i32p, _i64p]),
+    "nomad_tools_matrix_colptr": (_i64p, [_P]),
+    "nomad_tools_matrix_rowidx": (_i32p, [_P]),
+    "nomad_tools_matrix_val": (_f64p, [_P]),
+    "nomad_tools_matrix_free": (None, [_P]),
+    "nomad_tools_synth": (C.c_int, [C.POINTER(SynthSpec), C.POINTER(_P), C.POINTER(_P)]),
+    "nomad_tools_preset": (C.c_int, [C.c_int32, C.c_double, C.POINTER(SynthSpec)]),
+    "nomad_tools_preset_batch": (C.c_int, [C.c_int32, C.c_double, C.POINTER(SynthSpec), _i32p, _f64p]),
+    "nomad_tools_init_factors": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
+                                           _f64p, _f64p]),
+}
+
+_libs: dict[str, C.CDLL] = {}
+
+
+def _load(name: str, symbols: dict) -> C.CDLL:
+    if name in _libs:
+        return _libs[name]
+    path = LIBDIR / name
+    if not path.exists():
+        raise ImportError(f"{path} is not built; run `python -m nomad_amd.build` "
+                          "(or __graft_entry__.build())")
+    lib = C.CDLL(str(path), mode=C.RTLD_GLOBAL)
+    for sym, (res, args) in symbols.items():
+        fn = getattr(lib, sym)
+        fn.restype = res
+        fn.argtypes = args
+    _libs[name] = lib
+    return lib
+
+
+def hip() -> C.CDLL:
+    return _load("libnomad_hip.so", HIP_SYMBOLS)
+
+
+def tools() -> C.CDLL:
+    return _load("libnomad_tools.so", TOOLS_SYMBOLS)
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+# ---------------------------------------------------------------------------
+@dataclass
+class Csc:
+    """Global CSC of a rating matrix, rows ascending inside each column."""
+    m: int
+    n: int
+    colptr: np.ndarray  # int64[n+1]
+    rowidx: np.ndarray  # int32[nnz]
+    val: np.ndarray     # float64[nnz]
+
+    @property
+    def nnz(self) -> int:
+        return int(self.colptr[-1])
+
+    @staticmethod
+    def from_triplets(m: int, n: int, rows, cols, vals) -> "Csc":
+        rows = np.asarray(rows, np.int64)
+        cols = np.asarray(cols, np.int64)
+        vals = np.asarray(vals, np.float64)
+        order = np.lexsort((rows, cols))
+        colptr = np.zeros(n + 1, np.int64)
+        np.add.at(colptr, cols + 1, 1)
+        return Csc(m, n, np.cumsum(colptr), rows[order].astype(np.int32), vals[order].copy())
+
+    def to_triplets(self):
+        cols = np.repeat(np.arange(self.n, dtype=np.int64), np.diff(self.colptr))
+        return self.rowidx.astype(np.int64), cols, self.val
+
+
+def _tools_check(rc: int) -> None:
+    if rc != 0:
+        raise NomadError(rc, tools().nomad_tools_last_error().decode())
+
+
+def _matrix_to_csc(h) -> Csc:
+    L = tools()
+    m, n, nnz = C.c_int32(), C.c_int32(), C.c_int64()
+    _tools_check(L.nomad_tools_matrix_info(h, C.byref(m), C.byref(n), C.byref(nnz)))
+    colptr = np.ctypeslib.as_array(L.nomad_tools_matrix_colptr(h), (n.value + 1,)).copy()
+    if nnz.value:
+        rowidx = np.ctypeslib.as_array(L.nomad_tools_matrix_rowidx(h), (nnz.value,)).copy()
+        val = np.ctypeslib.as_array(L.nomad_tools_matrix_val(h), (nnz.value,)).copy()
+    else:
+        rowidx = np.zeros(0, np.int32)
+        val = np.zeros(0, np.float64)
+    return Csc(m.value, n.value, colptr, rowidx, val)
+
+
+def read_dat(path: str | os.PathLike) -> Csc:
+    h = C.c_void_p()
+    _tools_check(tools().nomad_tools_read_dat(str(path).encode(), C.byref(h)))
+    try:
+        return _matrix_to_csc(h)
+    finally:
+        tools().nomad_tools_matrix_free(h)
+
+
+def write_dat(path: str | os.PathLike, a: Csc, long_format: bool = False) -> None:
+    L = tools()
+    h = C.c_void_p()
+    colptr = np.ascontiguousarray(a.colptr, np.int64)
+    rowidx = np.ascontiguousarray(a.rowidx, np.int32)
+    val = np.ascontiguousarray(a.val, np.float64)
+    _tools_check(L.nomad_tools_matrix_from_csc(a.m, a.n, _ptr(colptr, C.c_int64), _ptr(rowidx, C.c_int32),
+                                               _ptr(val, C.c_double), C.byref(h)))
+    try:
+        _tools_check(L.nomad_tools_write_dat(str(path).encode(), h, int(long_format)))
+    finally:
+        L.nomad_tools_matrix_free(h)
+
+
+def preset(config: int, scale: float = 1.0) -> SynthSpec:
+    s = SynthSpec()
+    _tools_check(tools().nomad_tools_preset(config, scale, C.byref(s)))
+    return s
+
+
+def preset_batch(p: int, scale: float = 1.0):
+    s = SynthSpec()
+    k = C.c_int32()
+    reg = C.c_double()
+    _tools_check(tools().nomad_tools_preset_batch(p, scale, C.byref(s), C.byref(k), C.byref(reg)))
+    return s, k.value, reg.value
+
+
+def synth(spec: SynthSpec) -> tuple[Csc, Csc]:
+    L = tools()
+    a, b = C.c_void_p(), C.c_void_p()
+    _tools_check(L.nomad_tools_synth(C.byref(spec), C.byref(a), C.byref(b)))
+    try:
+        return _matrix_to_csc(a), _matrix_to_csc(b)
+    finally:
+        L.nomad_tools_matrix_free(a)
+        L.nomad_tools_matrix_free(b)
+
+
+def init_factors(m: int, n: int, k: int, nthreads: int, seed: int):
+    W = np.zeros((m, k), np.float64)
+    H = np.zeros((n, k), np.float64)
+    _tools_check(tools().nomad_tools_init_factors(m, n, k, nthreads, seed, _ptr(W, C.c_double),
+                                                  _ptr(H, C.c_double)))
+    return W, H
+
+
+# ---------------------------------------------------------------------------
+class Context:
+    """One factorisation on one GPU, driven through the C ABI."""
+
+    def __init__(self, m: int, n: int, k: int, *, storage: str | int = "fp32", mode: str | int = "async",
+                 rounding: str | int = "nearest", nworkers: int = 0, ptoken: int = 0, device: int = 0,
+                 lrate: float = 0.001, drate: float = 0.1, reg: float = 1.0, seed: int = 12345):
+        self._lib = hip()
+        p = Params()
+        p.m, p.n, p.k = m, n, k
+        p.storage = STORAGE_NAMES[storage] if isinstance(storage, str) else storage
+        p.mode = {"async": MODE_ASYNC, "strat": MODE_STRAT}[mode] if isinstance(mode, str) else mode
+        p.rounding = ({"nearest": ROUND_NEAREST, "stochastic": ROUND_STOCHASTIC}[rounding]
+                      if isinstance(rounding, str) else rounding)
+        p.nworkers, p.ptoken, p.device, p.flags = nworkers, ptoken, device, 0
+        p.lrate, p.drate, p.reg, p.seed = lrate, drate, reg, seed
+        self.params = p
+        self._h = C.c_void_p()
+        self._check(self._lib.nomad_create(C.byref(p), C.byref(self._h)))
+        self.m, self.n, self.k = m, n, k
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise NomadError(rc, self._lib.nomad_last_error().decode())
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.nomad_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def nworkers(self) -> int:
+        v = C.c_int32()
+        self._check(self._lib.nomad_get_nworkers(self._h, C.byref(v)))
+        return v.value
+
+    def upload(self, train: Csc, test: Csc | None = None) -> None:
+        for a, fn in ((train, self._lib.nomad_upload_train), (test, self._lib.nomad_upload_test)):
+            if a is None:
+                continue
+            colptr = np.ascontiguousarray(a.colptr, np.int64)
+            rowidx = np.ascontiguousarray(a.rowidx, np.int32)
+            val = np.ascontiguousarray(a.val, np.float64)
+            self._check(fn(self._h, _ptr(colptr, C.c_int64), _ptr(rowidx, C.c_int32), _ptr(val, C.c_double),
+                           a.nnz))
+
+    def set_factors(self, W: np.ndarray | None, H: np.ndarray | None) -> None:
+        Wc = None if W is None else np.ascontiguousarray(W, np.float64)
+        Hc = None if H is None else np.ascontiguousarray(H, np.float64)
+        self._check(self._lib.nomad_set_factors(self._h, None if Wc is None else _ptr(Wc, C.c_double),
+                                                None if Hc is None else _ptr(Hc, C.c_double)))
+
+    def get_factors(self):
+        W = np.zeros((self.m, self.k), np.float64)
+        H = np.zeros((self.n, self.k), np.float64)
+        self._check(self._lib.nomad_get_factors(self._h, _ptr(W, C.c_double), _ptr(H, C.c_double)))
+        return W, H
+
+    def train(self, seconds: float = 0.0, epochs: int = 0) -> TrainStats:
+        st = TrainStats()
+        self._check(self._lib.nomad_train(self._h, seconds, epochs, C.byref(st)))
+        return st
+
+    def eval(self) -> EvalStats:
+        st = EvalStats()
+        self._check(self._lib.nomad_eval(self._h, C.byref(st)))
+        return st

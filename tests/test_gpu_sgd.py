@@ -1,0 +1,183 @@
+"""GPU parity of the SGD kernels against the CPU oracle (through the C ABI).
+
+Bars:
+  * strat mode, all storages: bit-identical to the oracle's replay of the same
+    B x B schedule in the GPU's arithmetic order (ARITH_GPU), every epoch;
+  * strat mode, fp32: within 1e-5 relative (max-norm) of the oracle replaying
+    the reference's own arithmetic order (ARITH_REF, nb_main.cpp:528-538),
+    after each of several epochs;
+  * async mode: exactly nnz updates per epoch, every column back in place,
+    and test RMSE within 0.5% of the CPU async reference (T threads) after
+    the same number of epochs.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+HYP = dict(lrate=0.001, drate=0.1, reg=0.05)
+SR_SEED = lambda seed: (seed ^ (seed >> 32)) & 0xFFFFFFFF ^ 0xA5A5A5A5  # noqa: E731  (capi.hip sr_seed)
+
+
+def _ctx(lib, tr, te, k, **kw):
+    ctx = lib.Context(tr.m, tr.n, k, **{**HYP, **kw})
+    ctx.upload(tr, te)
+    return ctx
+
+
+def _relmax(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+@pytest.mark.parametrize("B", [1, 3, 8])
+@pytest.mark.parametrize("k", [16, 7])
+def test_strat_fp32_bitexact(gpu, tiny, B, k):
+    tr, te = tiny
+    W0, H0 = gpu.init_factors(tr.m, tr.n, k, 4, 12345)
+    with _ctx(gpu, tr, te, k, storage="fp32", mode="strat", nworkers=B) as ctx:
+        ctx.set_factors(W0, H0)
+        Wo, Ho = W0, H0
+        for ep in range(3):
+            ctx.train(epochs=1)
+            Wg, Hg = ctx.get_factors()
+            Wo, Ho = O.sgd_strat(tr, Wo, Ho, storage=O.FP32, arith=O.ARITH_GPU, B=B, epoch0=ep, epochs=1, **HYP)
+            assert np.array_equal(Wg, Wo), f"epoch {ep}: W differs, max {np.abs(Wg - Wo).max()}"
+            assert np.array_equal(Hg, Ho), f"epoch {ep}: H differs, max {np.abs(Hg - Ho).max()}"
+
+
+def test_strat_fp64_bitexact(gpu, tiny):
+    tr, te = tiny
+    k = 24
+    W0, H0 = gpu.init_factors(tr.m, tr.n, k, 2, 7)
+    with _ctx(gpu, tr, te, k, storage="fp64", mode="strat", nworkers=5) as ctx:
+        ctx.set_factors(W0, H0)
+        ctx.train(epochs=2)
+        Wg, Hg = ctx.get_factors()
+    Wo, Ho = O.sgd_strat(tr, W0, H0, storage=O.FP64, arith=O.ARITH_GPU, B=5, epochs=2, **HYP)
+    assert np.array_equal(Wg, Wo) and np.array_equal(Hg, Ho)
+
+
+@pytest.mark.parametrize("rounding", ["nearest", "stochastic"])
+def test_strat_bf16_bitexact(gpu, tiny, rounding):
+    tr, te = tiny
+    k = 32
+    seed = 12345
+    W0, H0 = gpu.init_factors(tr.m, tr.n, k, 4, seed)
+    with _ctx(gpu, tr, te, k, storage="bf16", mode="strat", nworkers=4, rounding=rounding, seed=seed) as ctx:
+        ctx.set_factors(W0, H0)
+        ctx.train(epochs=2)
+        Wg, Hg = ctx.get_factors()
+    r = O.ROUND_STOCHASTIC if rounding == "stochastic" else O.ROUND_NEAREST
+    Wo, Ho = O.sgd_strat(tr, W0, H0, storage=O.BF16, arith=O.ARITH_GPU, rounding=r, sr_seed=SR_SEED(seed), B=4,
+                         epochs=2, **HYP)
+    assert np.array_equal(Wg, Wo) and np.array_equal(Hg, Ho)
+
+
+def test_strat_fp32_vs_reference_order(gpu, small):
+    """GPU fused/tree arithmetic vs the reference's sequential order, per epoch."""
+    tr, te = small
+    k = 16
+    W0, H0 = gpu.init_factors(tr.m, tr.n, k, 4, 12345)
+    with _ctx(gpu, tr, te, k, storage="fp32", mode="strat", nworkers=8) as ctx:
+        ctx.set_factors(W0, H0)
+        Wo, Ho = W0, H0
+        for ep in range(4):
+            ctx.train(epochs=1)
+            Wg, Hg = ctx.get_factors()
+            Wo, Ho = O.sgd_strat(tr, Wo, Ho, storage=O.FP32, arith=O.ARITH_REF, B=8, epoch0=ep, epochs=1, **HYP)
+            assert _relmax(Wg, Wo) <= 1e-5, (ep, _relmax(Wg, Wo))
+            assert _relmax(Hg, Ho) <= 1e-5, (ep, _relmax(Hg, Ho))
+
+
+def test_serial_matches_reference_one_thread_order(gpu, tiny):
+    """B = 1 is the reference's --nthreads 1 order (columns cyclic, rows ascending)."""
+    tr, te = tiny
+    k = 8
+    W0, H0 = gpu.init_factors(tr.m, tr.n, k, 1, 99)
+    with _ctx(gpu, tr, te, k, storage="fp64", mode="strat", nworkers=1) as ctx:
+        ctx.set_factors(W0, H0)
+        ctx.train(epochs=2)
+        Wg, Hg = ctx.get_factors()
+    Wa, Ha, st = O.sgd_async(tr, W0, H0, storage=O.FP64, T=1, seed=99, epochs=2, **HYP)
+    assert st.updates == 2 * tr.nnz
+    assert _relmax(Wg, Wa) < 1e-12 and _relmax(Hg, Ha) < 1e-12
+
+
+@pytest.mark.parametrize("storage", ["fp32", "bf16", "fp64"])
+def test_async_counts_and_conservation(gpu, small, storage):
+    tr, te = small
+    k = 16
+    W0, H0 = gpu.init_factors(tr.m, tr.n, k, 4, 12345)
+    with _ctx(gpu, tr, te, k, storage=storage, mode="async", nworkers=64) as ctx:
+        ctx.set_factors(W0, H0)
+        total = 0
+        for ep in range(3):
+            st = ctx.train(epochs=1)
+            assert st.updates == tr.nnz, (ep, st.updates, tr.nnz)
+            assert st.epochs_done == ep + 1
+            total += st.updates
+        W, H = ctx.get_factors()
+    assert np.isfinite(W).all() and np.isfinite(H).all()
+    assert total == 3 * tr.nnz
+
+
+def test_async_rmse_matches_cpu_reference(gpu, small):
+    tr, te = small
+    k = 16
+    epochs = 8
+    W0, H0 = gpu.init_factors(tr.m, tr.n, k, 4, 12345)
+    with _ctx(gpu, tr, te, k, storage="fp32", mode="async", nworkers=128) as ctx:
+        ctx.set_factors(W0, H0)
+        ctx.train(epochs=epochs)
+        ev = ctx.eval()
+    gpu_rmse = np.sqrt(ev.test_sse / ev.test_count)
+    Wc, Hc, st = O.sgd_async(tr, W0, H0, storage=O.FP32, T=4, seed=12345, epochs=epochs, **HYP)
+    cpu_rmse = np.sqrt(O.sse(te, Wc, Hc, storage=O.FP32) / te.nnz)
+    assert abs(gpu_rmse - cpu_rmse) <= 0.005 * cpu_rmse, (gpu_rmse, cpu_rmse)
+
+
+def test_async_timeout_segments(gpu, small):
+    tr, te = small
+    k = 16
+    W0, H0 = gpu.init_factors(tr.m, tr.n, k, 4, 1)
+    with _ctx(gpu, tr, te, k, storage="fp32", mode="async", nworkers=32) as ctx:
+        ctx.set_factors(W0, H0)
+        st1 = ctx.train(seconds=0.05)
+        st2 = ctx.train(seconds=0.05)
+        ev = ctx.eval()
+    assert st1.updates > 0 and st2.updates > 0
+    assert np.isfinite(ev.test_sse)
+
+
+def test_eval_matches_oracle(gpu, small):
+    tr, te = small
+    k = 20
+    W0, H0 = gpu.init_factors(tr.m, tr.n, k, 3, 5)
+    for storage, ost in (("fp32", O.FP32), ("fp64", O.FP64)):
+        with _ctx(gpu, tr, te, k, storage=storage, mode="async", nworkers=16) as ctx:
+            ctx.set_factors(W0, H0)
+            ev = ctx.eval()
+            W, H = ctx.get_factors()
+        for a, got in ((tr, ev.train_sse), (te, ev.test_sse)):
+            ref = O.sse(a, W, H, storage=ost, arith=O.ARITH_GPU)
+            assert abs(got - ref) <= 1e-11 * ref, (storage, got, ref)
+            ref_order = O.sse(a, W, H, storage=ost, arith=O.ARITH_REF)
+            assert abs(got - ref_order) <= 1e-5 * ref_order
+
+
+def test_bad_arguments_raise(gpu):
+    with pytest.raises(gpu.NomadError):
+        gpu.Context(10, 10, 0)
+    with pytest.raises(gpu.NomadError):
+        gpu.Context(10, 10, 200)
+    ctx = gpu.Context(10, 10, 4, mode="strat", nworkers=2)
+    with pytest.raises(gpu.NomadError):
+        ctx.train(epochs=1)  # no data uploaded
+    colptr = np.ones(11, np.int64)
+    colptr[0] = 0
+    bad = gpu.Csc(10, 10, colptr, np.array([50], np.int32), np.array([1.0]))  # row 50 >= m
+    with pytest.raises(gpu.NomadError):
+        ctx.upload(bad)
+    ctx.close()

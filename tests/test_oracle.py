@@ -1,0 +1,232 @@
+"""CPU checks that pin the oracle against things other than itself: the C++
+standard's MT19937-64 definition, closed forms, hand-computed updates, numpy
+FP64 replays, finite differences and invariants."""
+import numpy as np
+import pytest
+
+from nomad_amd import lib
+from oracle import oracle as O
+
+HYP = dict(lrate=0.001, drate=0.1, reg=0.05)
+
+
+# --- an independent MT19937-64 (Matsumoto & Nishimura), pinned by the standard
+class MT64:
+    def __init__(self, seed):
+        self.mt = [0] * 312
+        self.mt[0] = seed & 0xFFFFFFFFFFFFFFFF
+        for i in range(1, 312):
+            x = self.mt[i - 1]
+            self.mt[i] = (6364136223846793005 * (x ^ (x >> 62)) + i) & 0xFFFFFFFFFFFFFFFF
+        self.i = 312
+
+    def __call__(self):
+        if self.i >= 312:
+            mt = self.mt
+            for i in range(312):
+                x = (mt[i] & 0xFFFFFFFF80000000) | (mt[(i + 1) % 312] & 0x7FFFFFFF)
+                mt[i] = mt[(i + 156) % 312] ^ (x >> 1) ^ (0xB5026F5AA96619E9 if x & 1 else 0)
+            self.i = 0
+        x = self.mt[self.i]
+        self.i += 1
+        x ^= (x >> 29) & 0x5555555555555555
+        x ^= (x << 17) & 0x71D67FFFEDA60000
+        x ^= (x << 37) & 0xFFF7EEE000000000
+        x ^= x >> 43
+        return x & 0xFFFFFFFFFFFFFFFF
+
+
+def test_mt64_standard_value():
+    g = MT64(5489)
+    for _ in range(9999):
+        g()
+    assert g() == 9981545732273789042  # [rand.predef] mt19937_64, 10000th output
+
+
+def _uniform(g, b):
+    # libstdc++ generate_canonical<double, 53> with a 64-bit engine: one draw,
+    # double(x) / 2^64, then a + (b - a) * u with a = 0
+    u = float(g()) / 2.0 ** 64
+    if u >= 1.0:
+        u = np.nextafter(1.0, 0.0)
+    return u * b + 0.0
+
+
+@pytest.mark.parametrize("T", [1, 3])
+def test_init_factors_streams(T):
+    m, n, k, seed = 10, 7, 5, 12345
+    W, H = O.init_factors(m, n, k, T, seed)
+    b = 1.0 / np.sqrt(k)
+    per = m // T + (m % T > 0)
+    for t in range(T):
+        g = MT64(seed + t + 1)
+        rows = range(t * per, min((t + 1) * per, m))
+        exp = [_uniform(g, b) for _ in range(len(rows) * k)]
+        assert np.array_equal(W[rows.start:rows.stop].ravel(), np.array(exp))
+    g = MT64(seed + 139)
+    assert np.array_equal(H.ravel(), np.array([_uniform(g, b) for _ in range(n * k)]))
+    # the product's initialiser draws the same streams
+    W2, H2 = lib.init_factors(m, n, k, T, seed)
+    assert np.array_equal(W, W2) and np.array_equal(H, H2)
+
+
+def test_init_perms_are_permutations():
+    P = O.init_perms(50, 6, 12345)
+    assert (np.sort(P, axis=1) == np.arange(6)).all()
+    assert np.array_equal(P, O.init_perms(50, 6, 12345))
+    assert not np.array_equal(P, O.init_perms(50, 6, 12346))
+
+
+def test_step_closed_form():
+    for c in range(5):
+        exp = 0.001 * 1.5 / (1.0 + 0.1 * (c + 1) ** 1.5)
+        assert O.step(O.FP64, 0.001, 0.1, c) == pytest.approx(exp, rel=1e-15)
+        assert O.step(O.FP32, 0.001, 0.1, c) == np.float32(
+            np.float64(np.float32(0.001)) * 1.5 / (1.0 + np.float64(np.float32(0.1)) * (c + 1) ** 1.5))
+
+
+def _one_rating(k=2):
+    # a 1 x 1 matrix with a single rating r = 3
+    return lib.Csc(1, 1, np.array([0, 1], np.int64), np.array([0], np.int32), np.array([3.0]))
+
+
+def test_one_rating_hand_computed():
+    a = _one_rating()
+    w = np.array([[0.5, -0.25]])
+    h = np.array([[0.2, 0.4]])
+    eta = 0.001 * 1.5 / (1.0 + 0.1 * 1.0)  # count 0
+    lam = 0.05
+    err = -3.0 + 0.2 * 0.5 + 0.4 * -0.25   # = -3.0
+    w_exp = w - eta * (err * h + lam * w)
+    h_exp = h - eta * (err * w + lam * h)
+    W1, H1 = O.sgd_strat(a, w, h, storage=O.FP64, arith=O.ARITH_REF, B=1, epochs=1, **HYP)
+    assert np.allclose(W1, w_exp, rtol=0, atol=1e-16) and np.allclose(H1, h_exp, rtol=0, atol=1e-16)
+    # both factors move from the *old* values (a dropped "tmp" would fail this)
+    assert not np.allclose(H1, h - eta * (err * W1 + lam * h))
+    W2, H2 = O.sgd_strat(a, w, h, storage=O.FP64, arith=O.ARITH_GPU, B=1, epochs=1, **HYP)
+    assert np.allclose(W2, W1, rtol=1e-15) and np.allclose(H2, H1, rtol=1e-15)
+
+
+def _numpy_serial(a, W, H, epochs, lrate, drate, reg):
+    W = W.copy()
+    H = H.copy()
+    for e in range(epochs):
+        eta = lrate * 1.5 / (1.0 + drate * (e + 1) ** 1.5)
+        for j in range(a.n):
+            for o in range(a.colptr[j], a.colptr[j + 1]):
+                i = a.rowidx[o]
+                err = W[i] @ H[j] - a.val[o]
+                wi = W[i].copy()
+                W[i] -= eta * (err * H[j] + reg * wi)
+                H[j] -= eta * (err * wi + reg * H[j])
+    return W, H
+
+
+def test_serial_replay_matches_numpy_fp64():
+    tr, _ = lib.synth(lib.preset(1, 0.05))
+    k = 6
+    W0, H0 = lib.init_factors(tr.m, tr.n, k, 1, 3)
+    Wn, Hn = _numpy_serial(tr, W0, H0, 2, **HYP)
+    Wo, Ho = O.sgd_strat(tr, W0, H0, storage=O.FP64, arith=O.ARITH_REF, B=1, epochs=2, **HYP)
+    assert np.allclose(Wo, Wn, rtol=1e-12, atol=1e-14) and np.allclose(Ho, Hn, rtol=1e-12, atol=1e-14)
+
+
+def test_strat_tiles_commute():
+    """Tiles of one sub-epoch touch disjoint rows and columns: B = 4 equals a
+    replay whose tiles run in another order -- checked by B=4 vs the numpy
+    replay of the same tile sequence."""
+    tr, _ = lib.synth(lib.preset(1, 0.05))
+    k, B = 4, 4
+    W0, H0 = lib.init_factors(tr.m, tr.n, k, 2, 9)
+    Wo, Ho = O.sgd_strat(tr, W0, H0, storage=O.FP64, arith=O.ARITH_REF, B=B, epochs=1, **HYP)
+    W, H = W0.copy(), H0.copy()
+    rpb = -(-tr.m // B)
+    cpb = -(-tr.n // B)
+    eta = HYP["lrate"] * 1.5 / (1.0 + HYP["drate"])
+    for s in range(B):
+        for b in reversed(range(B)):
+            cb = (b + s) % B
+            for j in range(cb * cpb, min(tr.n, (cb + 1) * cpb)):
+                for o in range(tr.colptr[j], tr.colptr[j + 1]):
+                    i = tr.rowidx[o]
+                    if not (b * rpb <= i < (b + 1) * rpb):
+                        continue
+                    err = W[i] @ H[j] - tr.val[o]
+                    wi = W[i].copy()
+                    W[i] -= eta * (err * H[j] + HYP["reg"] * wi)
+                    H[j] -= eta * (err * wi + HYP["reg"] * H[j])
+    assert np.allclose(Wo, W, rtol=1e-12, atol=1e-14) and np.allclose(Ho, H, rtol=1e-12, atol=1e-14)
+
+
+def test_gpu_order_close_to_reference_order():
+    tr, _ = lib.synth(lib.preset(1, 0.2))
+    k = 16
+    W0, H0 = lib.init_factors(tr.m, tr.n, k, 4, 12345)
+    Wr, Hr = O.sgd_strat(tr, W0, H0, storage=O.FP32, arith=O.ARITH_REF, B=4, epochs=2, **HYP)
+    Wg, Hg = O.sgd_strat(tr, W0, H0, storage=O.FP32, arith=O.ARITH_GPU, B=4, epochs=2, **HYP)
+    assert np.abs(Wr - Wg).max() <= 1e-5 * np.abs(Wr).max()
+    assert not np.array_equal(Wr, Wg)  # the two orders are really different
+
+
+def test_async_one_thread_is_serial():
+    tr, _ = lib.synth(lib.preset(1, 0.1))
+    W0, H0 = lib.init_factors(tr.m, tr.n, 8, 1, 4)
+    Wa, Ha, st = O.sgd_async(tr, W0, H0, storage=O.FP32, T=1, seed=4, epochs=3, **HYP)
+    Ws, Hs = O.sgd_strat(tr, W0, H0, storage=O.FP32, arith=O.ARITH_REF, B=1, epochs=3, **HYP)
+    assert st.updates == 3 * tr.nnz and st.min_cycles == 3
+    assert np.array_equal(Wa, Ws) and np.array_equal(Ha, Hs)
+
+
+def test_async_threads_update_count():
+    tr, _ = lib.synth(lib.preset(1, 0.1))
+    W0, H0 = lib.init_factors(tr.m, tr.n, 8, 4, 4)
+    _, _, st = O.sgd_async(tr, W0, H0, storage=O.FP64, T=4, seed=4, epochs=2, **HYP)
+    assert st.updates == 2 * tr.nnz and st.min_cycles == 2
+
+
+def test_sse_matches_numpy():
+    tr, te = lib.synth(lib.preset(1, 0.1))
+    W, H = lib.init_factors(tr.m, tr.n, 12, 2, 1)
+    r, c, v = te.to_triplets()
+    exp = float(np.sum((np.einsum("ij,ij->i", W[r], H[c]) - v) ** 2))
+    assert O.sse(te, W, H, storage=O.FP64) == pytest.approx(exp, rel=1e-12)
+    assert O.sse(te, W, H, storage=O.FP32) == pytest.approx(exp, rel=1e-5)
+
+
+def test_loss_grad_numpy_and_finite_differences():
+    tr, _ = lib.synth(lib.preset(4, 0.004))
+    rng = np.random.default_rng(0)
+    k, reg = 5, 0.1
+    W = rng.normal(size=(tr.m, k)) * 0.3
+    H = rng.normal(size=(tr.n, k)) * 0.3
+    L, gW, gH = O.loss_grad(tr, W, H, reg)
+    r, c, v = tr.to_triplets()
+    e = np.einsum("ij,ij->i", W[r], H[c]) - v
+    degr = np.bincount(r, minlength=tr.m)
+    degc = np.bincount(c, minlength=tr.n)
+    Lnp = np.sum(e ** 2) + reg * (np.sum(degr * np.sum(W ** 2, 1)) + np.sum(degc * np.sum(H ** 2, 1)))
+    assert L == pytest.approx(Lnp, rel=1e-12)
+    eps = 1e-6
+    for _ in range(20):
+        which = rng.integers(2)
+        M, G = (W, gW) if which == 0 else (H, gH)
+        i, d = rng.integers(M.shape[0]), rng.integers(k)
+        old = M[i, d]
+        M[i, d] = old + eps
+        Lp = O.loss_grad(tr, W, H, reg, want_grad=False)[0]
+        M[i, d] = old - eps
+        Lm = O.loss_grad(tr, W, H, reg, want_grad=False)[0]
+        M[i, d] = old
+        fd = (Lp - Lm) / (2 * eps)
+        assert fd == pytest.approx(G[i, d], rel=1e-5, abs=1e-6)
+
+
+def test_ccd_decreases_objective():
+    tr, _ = lib.synth(lib.preset(1, 0.1))
+    k = 4
+    rng = np.random.default_rng(1)
+    W = rng.random((tr.m, k))
+    H = np.zeros((tr.n, k))
+    _, _, obj = O.ccd(tr, W, H, 0.1, 3, 3)
+    o = obj[~np.isnan(obj)]
+    assert (np.diff(o) <= 1e-9 * o[:-1]).all()
