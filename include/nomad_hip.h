@@ -129,6 +129,13 @@ int nomad_eval(nomad_ctx* ctx, nomad_eval_stats* stats);
 /* Number of workers / strata the context actually uses (after auto sizing). */
 int nomad_get_nworkers(nomad_ctx* ctx, int32_t* nworkers);
 
+/* Device counters of the last async nomad_train call (no reference
+ * counterpart; diagnostics): out[0] updates, [1] (column, wave) visits,
+ * [2] idle polls, [3] ring claim attempts, [4] columns taken from rings,
+ * [5] columns leaving a workgroup, [6] sum over claim attempts of resident
+ * columns, [7] reserved.  n is the capacity of out (at most 8 are written). */
+int nomad_get_counters(nomad_ctx* ctx, int64_t* out, int32_t n);
+
 #ifdef __cplusplus
 }
 #endif

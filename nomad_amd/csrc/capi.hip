@@ -23,11 +23,12 @@
 
 using nomad_k::SgdArgs;
 using nomad_k::SseArgs;
+using nomad_k::TokInfo;
 using nomad_k::VisitDesc;
 
 template <int S>
 __global__ void sgd_strat_kernel(SgdArgs);
-template <int S>
+template <int S, int NW, int SLOTS>
 __global__ void sgd_async_kernel(SgdArgs);
 template <int S>
 __global__ void sse_kernel(SseArgs);
@@ -93,10 +94,15 @@ struct DevCsc {
 
 }  // namespace
 
+constexpr int kHierWaves = 16;  // waves (row sub-blocks) per workgroup of the async kernel
+constexpr int kStatWidth = nomad_k::kStatWidth;
+
 struct nomad_ctx {
     nomad_params p{};
-    int P = 0;  // workers (async) or strata (strat)
+    int P = 0;  // row blocks: workers (async) or strata (strat)
     int rpb = 0, cpb = 0, ldf = 0;
+    int units = 0;        // async: workgroups (route units)
+    int unit_rows = 0;    // rows per workgroup
     size_t esz = 4, rvsz = 4;
     uint32_t sr_seed = 0;
     hipStream_t stream = nullptr;
@@ -121,9 +127,7 @@ struct nomad_ctx {
     std::vector<VisitDesc> h_visits;
     int64_t* d_vptr = nullptr;
     VisitDesc* d_visits = nullptr;
-    int32_t* d_vcnt = nullptr;
-    int32_t* d_tok_pos = nullptr;
-    int32_t* d_tok_cycle = nullptr;
+    TokInfo* d_tok = nullptr;
     uint64_t* d_qslot = nullptr;
     uint32_t* d_qtail = nullptr;
     uint32_t* d_qhead = nullptr;
@@ -132,6 +136,7 @@ struct nomad_ctx {
     uint32_t* d_timed_out = nullptr;
     int64_t* d_stats = nullptr;
     std::vector<int64_t> stats_prev;
+    int64_t last_stats[nomad_k::kStatWidth] = {0};
     int32_t ncols_active = 0;
     std::mt19937_64 rng_route;   // initial permutations (nb_main.cpp:79)
     std::mt19937_64 rng_reperm;  // re-permutation after a segment (nb_init.cpp:219, nb_main.cpp:316)
@@ -151,9 +156,7 @@ struct nomad_ctx {
         dev_free(d_partial);
         dev_free(d_vptr);
         dev_free(d_visits);
-        dev_free(d_vcnt);
-        dev_free(d_tok_pos);
-        dev_free(d_tok_cycle);
+        dev_free(d_tok);
         dev_free(d_qslot);
         dev_free(d_qtail);
         dev_free(d_qhead);
@@ -242,48 +245,34 @@ int upload_csc(nomad_ctx* c, DevCsc& d, const int64_t* colptr, const int32_t* ro
 // (Code/nomad/pool.cpp:32-38) with P workers in place of the reference's
 // threads.  Workers with no rating of the column are skipped: their visit in
 // the reference is an empty loop.
-void build_routes(nomad_ctx* c, std::mt19937_64& rng, const std::vector<int32_t>* old_counts,
-                  std::vector<int32_t>& counts_out) {
-    const int32_t n = c->p.n, P = c->P;
+void build_routes(nomad_ctx* c, std::mt19937_64& rng) {
+    const int32_t n = c->p.n, P = c->units;
     std::vector<int32_t> perm(P), rank(P);
     std::vector<VisitDesc> runs;
-    std::vector<int32_t> run_cnt;
     const bool first = c->h_vptr.empty();
     std::vector<int64_t> vptr(size_t(n) + 1, 0);
     std::vector<VisitDesc> visits;
     visits.reserve(c->h_visits.size());
-    counts_out.clear();
-    counts_out.reserve(c->h_visits.size());
     for (int32_t j = 0; j < n; ++j) {
         for (int32_t q = 0; q < P; ++q) perm[q] = q;
         std::shuffle(perm.begin(), perm.end(), rng);
         for (int32_t q = 0; q < P; ++q) rank[perm[q]] = q;
         runs.clear();
-        run_cnt.clear();
         if (first) {
             const int64_t c0 = c->h_colptr[j], c1 = c->h_colptr[j + 1];
             for (int64_t e = c0; e < c1;) {
-                const int32_t wk = c->h_rowidx[e] / c->rpb;
+                const int32_t wk = c->h_rowidx[e] / c->unit_rows;
                 int64_t f = e;
-                while (f < c1 && c->h_rowidx[f] / c->rpb == wk) ++f;
+                while (f < c1 && c->h_rowidx[f] / c->unit_rows == wk) ++f;
                 runs.push_back(VisitDesc{e, int32_t(f - e), wk});
-                run_cnt.push_back(0);
                 e = f;
             }
         } else {
-            for (int64_t v = c->h_vptr[j]; v < c->h_vptr[j + 1]; ++v) {
-                runs.push_back(c->h_visits[v]);
-                run_cnt.push_back((*old_counts)[v]);
-            }
+            for (int64_t v = c->h_vptr[j]; v < c->h_vptr[j + 1]; ++v) runs.push_back(c->h_visits[v]);
         }
-        std::vector<int32_t> order(runs.size());
-        std::iota(order.begin(), order.end(), 0);
-        std::sort(order.begin(), order.end(),
-                  [&](int32_t a, int32_t b) { return rank[runs[a].worker] < rank[runs[b].worker]; });
-        for (int32_t o : order) {
-            visits.push_back(runs[o]);
-            counts_out.push_back(run_cnt[o]);
-        }
+        std::sort(runs.begin(), runs.end(),
+                  [&](const VisitDesc& a, const VisitDesc& b) { return rank[a.worker] < rank[b.worker]; });
+        visits.insert(visits.end(), runs.begin(), runs.end());
         vptr[j + 1] = int64_t(visits.size());
     }
     c->h_vptr.swap(vptr);
@@ -291,30 +280,34 @@ void build_routes(nomad_ctx* c, std::mt19937_64& rng, const std::vector<int32_t>
     c->V = int64_t(c->h_visits.size());
 }
 
+// Routes, rings and token records for the next launch.  Every call starts
+// every active column at the first workgroup of its route; from the second
+// call on the routes are re-drawn first, as the reference re-permutes every
+// column after each evaluation (nb_main.cpp:313-321).
 int setup_async(nomad_ctx* c) {
-    std::vector<int32_t> counts;
     int rc;
+    const int32_t n = c->p.n, P = c->units;
+    std::vector<TokInfo> tok(static_cast<size_t>(n));
     if (!c->routes_ready) {
         c->h_vptr.clear();
         c->h_visits.clear();
-        build_routes(c, c->rng_route, nullptr, counts);
-        const int32_t n = c->p.n, P = c->P;
-        // ring capacity: at least the number of columns that can ever be queued
-        // at one worker (its distinct columns) + 1
-        std::vector<int32_t> per_worker(P, 0);
+        build_routes(c, c->rng_route);
+        // ring capacity: more than the number of columns that can ever be
+        // queued at one workgroup (its distinct columns)
+        std::vector<int32_t> per_unit(P, 0);
         c->ncols_active = 0;
         for (int32_t j = 0; j < n; ++j) {
             if (c->h_vptr[j + 1] > c->h_vptr[j]) ++c->ncols_active;
-            for (int64_t v = c->h_vptr[j]; v < c->h_vptr[j + 1]; ++v) per_worker[c->h_visits[v].worker]++;
+            for (int64_t v = c->h_vptr[j]; v < c->h_vptr[j + 1]; ++v) per_unit[c->h_visits[v].worker]++;
         }
-        const int32_t maxq = P ? *std::max_element(per_worker.begin(), per_worker.end()) : 0;
+        const int32_t maxq = P ? *std::max_element(per_unit.begin(), per_unit.end()) : 0;
         c->qlog2 = 1;
         while ((int64_t(1) << c->qlog2) < int64_t(maxq) + 1) ++c->qlog2;
+        if (c->V >= (int64_t(1) << 31) || c->tr.nnz >= (int64_t(1) << 32))
+            return fail(NOMAD_EINVAL, "async mode: more than 2^31 visits or 2^32 ratings");
         dev_free(c->d_vptr);
         dev_free(c->d_visits);
-        dev_free(c->d_vcnt);
-        dev_free(c->d_tok_pos);
-        dev_free(c->d_tok_cycle);
+        dev_free(c->d_tok);
         dev_free(c->d_qslot);
         dev_free(c->d_qtail);
         dev_free(c->d_qhead);
@@ -323,39 +316,40 @@ int setup_async(nomad_ctx* c) {
         dev_free(c->d_stats);
         if ((rc = dev_alloc(&c->d_vptr, size_t(n) + 1))) return rc;
         if ((rc = dev_alloc(&c->d_visits, size_t(c->V)))) return rc;
-        if ((rc = dev_alloc(&c->d_vcnt, size_t(c->V)))) return rc;
-        if ((rc = dev_alloc(&c->d_tok_pos, size_t(n)))) return rc;
-        if ((rc = dev_alloc(&c->d_tok_cycle, size_t(n)))) return rc;
+        if ((rc = dev_alloc(&c->d_tok, size_t(n)))) return rc;
         if ((rc = dev_alloc(&c->d_qslot, size_t(P) << c->qlog2))) return rc;
         if ((rc = dev_alloc(&c->d_qtail, size_t(P)))) return rc;
         if ((rc = dev_alloc(&c->d_qhead, size_t(P)))) return rc;
         if ((rc = dev_alloc(&c->d_done, 1))) return rc;
         if ((rc = dev_alloc(&c->d_timed_out, 1))) return rc;
-        if ((rc = dev_alloc(&c->d_stats, size_t(P) * 3))) return rc;
-        NOMAD_HIP_TRY(hipMemset(c->d_tok_cycle, 0, size_t(n) * 4));
+        if ((rc = dev_alloc(&c->d_stats, size_t(P) * kStatWidth))) return rc;
         NOMAD_HIP_TRY(hipMemset(c->d_qslot, 0, (size_t(P) << c->qlog2) * 8));
         NOMAD_HIP_TRY(hipMemset(c->d_qtail, 0, size_t(P) * 4));
         NOMAD_HIP_TRY(hipMemset(c->d_qhead, 0, size_t(P) * 4));
-        NOMAD_HIP_TRY(hipMemset(c->d_stats, 0, size_t(P) * 3 * 8));
-        c->stats_prev.assign(size_t(P) * 3, 0);
+        NOMAD_HIP_TRY(hipMemset(c->d_stats, 0, size_t(P) * kStatWidth * 8));
+        c->stats_prev.assign(size_t(P) * kStatWidth, 0);
+        for (auto& t : tok) t = TokInfo{0, 0, 0, 0};
         c->routes_ready = true;
         c->segments = 0;
         c->cycle_base = 0;
     } else {
-        // Re-permute every column before a new segment, as the reference does
-        // after each evaluation (nb_main.cpp:313-321); visit counts move with
-        // their (column, worker) entry.
-        std::vector<int32_t> old(static_cast<size_t>(c->V));
-        NOMAD_HIP_TRY(hipMemcpy(old.data(), c->d_vcnt, size_t(c->V) * 4, hipMemcpyDeviceToHost));
-        build_routes(c, c->rng_reperm, &old, counts);
+        NOMAD_HIP_TRY(hipMemcpy(tok.data(), c->d_tok, size_t(n) * sizeof(TokInfo), hipMemcpyDeviceToHost));
+        build_routes(c, c->rng_reperm);
         // drop whatever is still queued: head = tail on every ring
-        NOMAD_HIP_TRY(hipMemcpy(c->d_qhead, c->d_qtail, size_t(c->P) * 4, hipMemcpyDeviceToDevice));
+        NOMAD_HIP_TRY(hipMemcpy(c->d_qhead, c->d_qtail, size_t(P) * 4, hipMemcpyDeviceToDevice));
+    }
+    for (int32_t j = 0; j < n; ++j) {
+        if (c->h_vptr[j + 1] > c->h_vptr[j]) {
+            const VisitDesc& v = c->h_visits[c->h_vptr[j]];
+            tok[j].start = uint32_t(v.start);
+            tok[j].len = v.len;
+        }
+        tok[j].pos = 0;
     }
     NOMAD_HIP_TRY(hipMemcpy(c->d_vptr, c->h_vptr.data(), c->h_vptr.size() * 8, hipMemcpyHostToDevice));
     NOMAD_HIP_TRY(hipMemcpy(c->d_visits, c->h_visits.data(), size_t(c->V) * sizeof(VisitDesc),
                             hipMemcpyHostToDevice));
-    NOMAD_HIP_TRY(hipMemcpy(c->d_vcnt, counts.data(), size_t(c->V) * 4, hipMemcpyHostToDevice));
-    NOMAD_HIP_TRY(hipMemset(c->d_tok_pos, 0, size_t(c->p.n) * 4));
+    NOMAD_HIP_TRY(hipMemcpy(c->d_tok, tok.data(), size_t(n) * sizeof(TokInfo), hipMemcpyHostToDevice));
     return NOMAD_OK;
 }
 
@@ -365,7 +359,7 @@ SgdArgs base_args(nomad_ctx* c) {
     a.n = c->p.n;
     a.k = c->p.k;
     a.ldf = c->ldf;
-    a.nblocks = c->P;
+    a.nblocks = c->p.mode == NOMAD_MODE_ASYNC ? c->units : c->P;
     a.rpb = c->rpb;
     a.cpb = c->cpb;
     a.round_mode = c->p.rounding;
@@ -391,7 +385,8 @@ hipError_t launch_strat(const SgdArgs& a, hipStream_t st) {
 
 template <int S>
 hipError_t launch_async(const SgdArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(sgd_async_kernel<S>, dim3(a.nblocks), dim3(64), 0, st, a);
+    constexpr int slots = S == NOMAD_STORE_FP64 ? 64 : 128;
+    hipLaunchKernelGGL((sgd_async_kernel<S, kHierWaves, slots>), dim3(a.nblocks), dim3(kHierWaves * 64), 0, st, a);
     return hipGetLastError();
 }
 
@@ -420,7 +415,7 @@ double watchdog_seconds() {
 int train_async(nomad_ctx* c, double seconds, int64_t epochs, nomad_train_stats* st) {
     int rc = setup_async(c);
     if (rc) return rc;
-    const int32_t n = c->p.n, P = c->P;
+    const int32_t n = c->p.n, P = c->units;
     // seed every active column at the first worker of its (new) route, in
     // column order
     std::vector<int64_t> off(size_t(P) + 1, 0);
@@ -457,9 +452,7 @@ int train_async(nomad_ctx* c, double seconds, int64_t epochs, nomad_train_stats*
     SgdArgs a = base_args(c);
     a.vptr = c->d_vptr;
     a.visits = c->d_visits;
-    a.vcnt = c->d_vcnt;
-    a.tok_pos = c->d_tok_pos;
-    a.tok_cycle = c->d_tok_cycle;
+    a.tok = c->d_tok;
     a.qslot = c->d_qslot;
     a.qtail = c->d_qtail;
     a.qhead = c->d_qhead;
@@ -491,25 +484,24 @@ int train_async(nomad_ctx* c, double seconds, int64_t epochs, nomad_train_stats*
     uint32_t done = 0, timed_out = 0;
     NOMAD_HIP_TRY(hipMemcpy(&done, c->d_done, 4, hipMemcpyDeviceToHost));
     NOMAD_HIP_TRY(hipMemcpy(&timed_out, c->d_timed_out, 4, hipMemcpyDeviceToHost));
-    std::vector<int64_t> stats(size_t(P) * 3);
+    std::vector<int64_t> stats(size_t(P) * kStatWidth);
     NOMAD_HIP_TRY(hipMemcpy(stats.data(), c->d_stats, stats.size() * 8, hipMemcpyDeviceToHost));
-    int64_t upd = 0, vis = 0, fails = 0;
-    for (int32_t q = 0; q < P; ++q) {
-        upd += stats[3 * q] - c->stats_prev[3 * q];
-        vis += stats[3 * q + 1] - c->stats_prev[3 * q + 1];
-        fails += stats[3 * q + 2] - c->stats_prev[3 * q + 2];
-    }
+    int64_t sum[kStatWidth] = {0};
+    for (int32_t q = 0; q < P; ++q)
+        for (int i = 0; i < kStatWidth; ++i) sum[i] += stats[size_t(q) * kStatWidth + i] - c->stats_prev[size_t(q) * kStatWidth + i];
     c->stats_prev.swap(stats);
+    for (int i = 0; i < kStatWidth; ++i) c->last_stats[i] = sum[i];
+    const int64_t upd = sum[nomad_k::kStatUpdates], vis = sum[nomad_k::kStatVisits], fails = sum[nomad_k::kStatIdle];
     ++c->segments;
     const bool all_parked = int32_t(done) >= c->ncols_active;
     if (all_parked && epochs > 0) {
         c->cycle_base = target;
     } else {
-        std::vector<int32_t> cyc(static_cast<size_t>(n));
-        NOMAD_HIP_TRY(hipMemcpy(cyc.data(), c->d_tok_cycle, size_t(n) * 4, hipMemcpyDeviceToHost));
+        std::vector<TokInfo> tok(static_cast<size_t>(n));
+        NOMAD_HIP_TRY(hipMemcpy(tok.data(), c->d_tok, size_t(n) * sizeof(TokInfo), hipMemcpyDeviceToHost));
         int64_t mn = INT64_MAX;
         for (int32_t j = 0; j < n; ++j)
-            if (c->h_vptr[j + 1] > c->h_vptr[j]) mn = std::min<int64_t>(mn, cyc[j]);
+            if (c->h_vptr[j + 1] > c->h_vptr[j]) mn = std::min<int64_t>(mn, tok[j].cycle);
         c->cycle_base = mn == INT64_MAX ? c->cycle_base : mn;
     }
     if (st) {
@@ -659,20 +651,30 @@ int nomad_create(const nomad_params* p, nomad_ctx** out) {
 
     int P = p->nworkers;
     if (p->mode == NOMAD_MODE_ASYNC) {
-        if (P == 0) {
-            int cus = 0, per_cu = 0;
-            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess ||
-                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sgd_async_kernel<NOMAD_STORE_FP32>, 64, 0) !=
-                    hipSuccess)
-                return bail(fail(NOMAD_EHIP, "nomad_create: occupancy query failed"));
-            P = cus * std::max(1, std::min(per_cu, 4));
-        }
+        // workgroups of kHierWaves waves; all must be co-resident
+        int cus = 0, per_cu = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess)
+            return bail(fail(NOMAD_EHIP, "nomad_create: device query failed"));
+        hipError_t oe = dispatch_storage(p->storage, [&](auto t) {
+            constexpr int S = decltype(t)::value;
+            constexpr int slots = S == NOMAD_STORE_FP64 ? 64 : 128;
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sgd_async_kernel<S, kHierWaves, slots>,
+                                                                kHierWaves * 64, 0);
+        });
+        if (oe != hipSuccess || per_cu < 1) return bail(fail(NOMAD_EHIP, "nomad_create: occupancy query failed"));
+        const int gmax = cus * per_cu;
+        int G = P > 0 ? (P + kHierWaves - 1) / kHierWaves : gmax;
+        G = std::max(1, std::min(G, gmax));
+        G = std::max(1, std::min(G, (p->m + kHierWaves - 1) / kHierWaves));
+        P = G * kHierWaves;
+        c->units = G;
     } else if (P == 0) {
         P = 64;
     }
-    P = std::max(1, std::min(P, p->m));
+    if (p->mode == NOMAD_MODE_STRAT) P = std::max(1, std::min(P, p->m));
     c->P = P;
     c->rpb = p->m / P + ((p->m % P) > 0 ? 1 : 0);  // nb_data.cpp:83
+    if (p->mode == NOMAD_MODE_ASYNC) c->unit_rows = c->rpb * kHierWaves;
     if (p->mode == NOMAD_MODE_STRAT) c->cpb = p->n / P + ((p->n % P) > 0 ? 1 : 0);
 
     int rc;
@@ -696,6 +698,12 @@ int nomad_destroy(nomad_ctx* c) {
 int nomad_get_nworkers(nomad_ctx* c, int32_t* nworkers) {
     if (!c || !nworkers) return fail(NOMAD_EINVAL, "nomad_get_nworkers: null argument");
     *nworkers = c->P;
+    return NOMAD_OK;
+}
+
+int nomad_get_counters(nomad_ctx* c, int64_t* out, int32_t n) {
+    if (!c || !out || n < 0) return fail(NOMAD_EINVAL, "nomad_get_counters: bad argument");
+    for (int32_t i = 0; i < std::min<int32_t>(n, kStatWidth); ++i) out[i] = c->last_stats[i];
     return NOMAD_OK;
 }
 

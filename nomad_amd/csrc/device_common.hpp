@@ -106,6 +106,28 @@ __device__ __forceinline__ uint32_t sr_bits(uint32_t seed, uint32_t a, uint32_t 
 // data-dependent select after it, so several rows can be in flight.
 __device__ __align__(16) static double g_zero_pairs[2 * kWave];  // zero-initialised, never written
 
+// Device-coherent accesses: relaxed agent-scope atomics, i.e. loads and
+// stores with the sc1 bit, which are coherent across the XCDs' separate L2s
+// without the L2 writeback / invalidate that an agent-scope fence costs.
+template <class T>
+__device__ __forceinline__ T ld_dc(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void st_dc(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits until every memory operation this wave has issued has completed
+// (s_waitcnt vmcnt(0) expcnt(0) lgkmcnt(0)); with device-coherent stores that
+// means they are visible to every XCD.  The signal fences keep the compiler
+// from moving memory operations across the wait.
+__device__ __forceinline__ void wait_all_mem() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 template <int STORE>
 struct Store;
 
@@ -127,6 +149,19 @@ struct Store<NOMAD_STORE_FP32> {
                                                  int) {
         if (lane < npairs) reinterpret_cast<float2*>(row)[lane] = make_float2(p.x, p.y);
     }
+    // device-coherent (sc1) variants for rows that move between workers
+    static __device__ __forceinline__ pair load_dc(const elem* row, int lane, int npairs) {
+        const uint64_t* src = lane < npairs ? reinterpret_cast<const uint64_t*>(row)
+                                            : reinterpret_cast<const uint64_t*>(g_zero_pairs);
+        const uint64_t u = ld_dc(src + lane);
+        return pair{__uint_as_float(uint32_t(u)), __uint_as_float(uint32_t(u >> 32))};
+    }
+    static __device__ __forceinline__ void store_dc(elem* row, int lane, int npairs, pair p, uint32_t,
+                                                    int) {
+        if (lane < npairs)
+            st_dc(reinterpret_cast<uint64_t*>(row) + lane,
+                  (uint64_t(__float_as_uint(p.y)) << 32) | __float_as_uint(p.x));
+    }
 };
 
 template <>
@@ -146,6 +181,20 @@ struct Store<NOMAD_STORE_FP64> {
     static __device__ __forceinline__ void store(elem* row, int lane, int npairs, pair p, uint32_t,
                                                  int) {
         if (lane < npairs) reinterpret_cast<double2*>(row)[lane] = make_double2(p.x, p.y);
+    }
+    static __device__ __forceinline__ pair load_dc(const elem* row, int lane, int npairs) {
+        const uint64_t* src = lane < npairs ? reinterpret_cast<const uint64_t*>(row)
+                                            : reinterpret_cast<const uint64_t*>(g_zero_pairs);
+        const uint64_t a = ld_dc(src + 2 * lane);
+        const uint64_t b = ld_dc(src + 2 * lane + 1);
+        return pair{__builtin_bit_cast(double, a), __builtin_bit_cast(double, b)};
+    }
+    static __device__ __forceinline__ void store_dc(elem* row, int lane, int npairs, pair p, uint32_t,
+                                                    int) {
+        if (lane < npairs) {
+            st_dc(reinterpret_cast<uint64_t*>(row) + 2 * lane, __builtin_bit_cast(uint64_t, p.x));
+            st_dc(reinterpret_cast<uint64_t*>(row) + 2 * lane + 1, __builtin_bit_cast(uint64_t, p.y));
+        }
     }
 };
 
@@ -175,6 +224,24 @@ struct Store<NOMAD_STORE_BF16> {
             hi = f32_to_bf16_rne(p.y);
         }
         if (lane < npairs) reinterpret_cast<uint32_t*>(row)[lane] = lo | (hi << 16);
+    }
+    static __device__ __forceinline__ pair load_dc(const elem* row, int lane, int npairs) {
+        const uint32_t* src = lane < npairs ? reinterpret_cast<const uint32_t*>(row)
+                                            : reinterpret_cast<const uint32_t*>(g_zero_pairs);
+        const uint32_t u = ld_dc(src + lane);
+        return pair{bf16_lo(u), bf16_hi(u)};
+    }
+    static __device__ __forceinline__ void store_dc(elem* row, int lane, int npairs, pair p, uint32_t rnd,
+                                                    int round_mode) {
+        uint32_t lo, hi;
+        if (round_mode == NOMAD_ROUND_STOCHASTIC) {
+            lo = f32_to_bf16_sr(p.x, rnd);
+            hi = f32_to_bf16_sr(p.y, rnd >> 16);
+        } else {
+            lo = f32_to_bf16_rne(p.x);
+            hi = f32_to_bf16_rne(p.y);
+        }
+        if (lane < npairs) st_dc(reinterpret_cast<uint32_t*>(row) + lane, lo | (hi << 16));
     }
 };
 

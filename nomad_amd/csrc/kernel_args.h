@@ -13,6 +13,21 @@ struct VisitDesc {
     int32_t worker;
 };
 
+// Per-column token state kept in HBM between holders: the CSC range of the
+// column's next visit, its position in the column's route and the number of
+// completed route cycles (the step-size counter).  Written with
+// device-coherent stores before the column is published on a ring.
+struct alignas(16) TokInfo {
+    uint32_t start;
+    int32_t len;
+    int32_t pos;
+    int32_t cycle;
+};
+
+// per-workgroup counters of the async kernel
+enum AsyncStat : int { kStatUpdates = 0, kStatVisits, kStatIdle, kStatFetches, kStatFetched, kStatExits,
+                       kStatResident, kStatWidth = 8 };
+
 struct SgdArgs {
     // problem
     int32_t m, n, k, ldf;
@@ -31,20 +46,18 @@ struct SgdArgs {
     double lrate, drate;  // for counts past the table
     double reg;
 
-    // async schedule
+    // async schedule (two-level: workgroups of waves)
     const int64_t* vptr;       // n + 1 offsets into visits
-    const VisitDesc* visits;   // per column, in route order
-    int32_t* vcnt;             // visit counts (step-size counter) per visit entry
-    int32_t* tok_pos;          // per column: index of its next visit in its route
-    int32_t* tok_cycle;        // per column: completed route cycles
+    const VisitDesc* visits;   // per column, in route order; .worker = workgroup
+    TokInfo* tok;              // per column: its next visit and completed cycles
     uint64_t* qslot;           // nblocks x (1 << qlog2) ring slots: (lap+1) << 32 | column
-    uint32_t* qtail;           // per worker, producers' reservation counter
-    uint32_t* qhead;           // per worker, consumer position (persisted across launches)
+    uint32_t* qtail;           // per workgroup, producers' reservation counter
+    uint32_t* qhead;           // per workgroup, consumer position (persisted across launches)
     uint32_t* done;            // columns parked after reaching target_cycle
-    int64_t* stats;            // per worker: updates, visits, failed pops
-    uint32_t* timed_out;       // set when a worker stopped on the deadline
+    int64_t* stats;            // per workgroup: kStatWidth counters
+    uint32_t* timed_out;       // set when a workgroup stopped on the deadline
     int32_t qlog2;
-    int32_t target_cycle;      // park a column when tok_cycle reaches this
+    int32_t target_cycle;      // park a column when its cycle count reaches this
     int32_t ncols_active;      // columns circulating in this launch
     int32_t pad0;
     uint64_t deadline;         // wall_clock64 ticks after its start at which a worker stops

@@ -80,6 +80,7 @@ HIP_SYMBOLS = {
     "nomad_train": (C.c_int, [_P, C.c_double, C.c_int64, C.POINTER(TrainStats)]),
     "nomad_eval": (C.c_int, [_P, C.POINTER(EvalStats)]),
     "nomad_get_nworkers": (C.c_int, [_P, _i32p]),
+    "nomad_get_counters": (C.c_int, [_P, _i64p, C.c_int32]),
 }
 
 TOOLS_SYMBOLS = {
@@ -282,6 +283,12 @@ class Context:
         v = C.c_int32()
         self._check(self._lib.nomad_get_nworkers(self._h, C.byref(v)))
         return v.value
+
+    def counters(self) -> dict:
+        out = np.zeros(8, np.int64)
+        self._check(self._lib.nomad_get_counters(self._h, _ptr(out, C.c_int64), 8))
+        names = ["updates", "visits", "idle_polls", "claims", "claimed", "exits", "resident_sum", "reserved"]
+        return dict(zip(names, out.tolist()))
 
     def upload(self, train: Csc, test: Csc | None = None) -> None:
         for a, fn in ((train, self._lib.nomad_upload_train), (test, self._lib.nomad_upload_test)):

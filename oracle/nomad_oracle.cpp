@@ -440,19 +440,29 @@ int async_run(int T, int m, int n, int k, const int64_t* colptr, const int32_t* 
         const int max_row = std::min(per * (t + 1), m);
         P.rows = std::max(0, max_row - P.min_row);
         P.off.assign(size_t(n) + 1, 0);
-        for (int j = 0; j < n; ++j) {
-            for (int64_t o = colptr[j]; o < colptr[j + 1]; ++o) {
-                const int i = rowidx[o];
-                if (i >= P.min_row && i < max_row) {
-                    P.ri.push_back(i - P.min_row);
-                    P.rv.push_back(A(val[o]));
-                }
-            }
-            P.off[j + 1] = int64_t(P.ri.size());
-        }
         P.w.resize(size_t(P.rows) * k);
         for (int64_t e = 0; e < int64_t(P.rows) * k; ++e) P.w[e] = A(W[int64_t(P.min_row) * k + e]);
         P.counts.assign(n, 0);
+    }
+    // one pass: count, then place (rows stay ascending inside a column)
+    for (int j = 0; j < n; ++j)
+        for (int64_t o = colptr[j]; o < colptr[j + 1]; ++o) parts[rowidx[o] / per].off[j + 1]++;
+    for (int t = 0; t < T; ++t) {
+        Part& P = parts[t];
+        for (int j = 0; j < n; ++j) P.off[j + 1] += P.off[j];
+        P.ri.resize(P.off[n]);
+        P.rv.resize(P.off[n]);
+    }
+    {
+        std::vector<std::vector<int64_t>> fill(T);
+        for (int t = 0; t < T; ++t) fill[t].assign(parts[t].off.begin(), parts[t].off.end() - 1);
+        for (int j = 0; j < n; ++j)
+            for (int64_t o = colptr[j]; o < colptr[j + 1]; ++o) {
+                const int t = rowidx[o] / per;
+                const int64_t d = fill[t][j]++;
+                parts[t].ri[d] = rowidx[o] - parts[t].min_row;
+                parts[t].rv[d] = A(val[o]);
+            }
     }
     std::vector<Token> toks(n);
     std::vector<std::vector<A>> hv(n, std::vector<A>(k));
