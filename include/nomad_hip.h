@@ -133,7 +133,13 @@ int nomad_get_nworkers(nomad_ctx* ctx, int32_t* nworkers);
  * counterpart; diagnostics): out[0] updates, [1] (column, wave) visits,
  * [2] idle polls, [3] ring claim attempts, [4] columns taken from rings,
  * [5] columns leaving a workgroup, [6] sum over claim attempts of resident
- * columns, [7] reserved.  n is the capacity of out (at most 8 are written). */
+ * columns, [7] claim attempts that found published entries queued behind an
+ * unpublished one, [8] wall-clock ticks spent in visits, [9] ticks spent
+ * handing columns to the next workgroup, [10] ticks spent claiming columns,
+ * [11] ticks inside the SGD chains, [12] ticks between a column's
+ * publication on a ring and its claim, [13..15] reserved.  Ticks run at the
+ * device's constant wall-clock rate.  n is the capacity of out (at most 16
+ * are written). */
 int nomad_get_counters(nomad_ctx* ctx, int64_t* out, int32_t n);
 
 #ifdef __cplusplus

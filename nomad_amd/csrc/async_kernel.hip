@@ -117,7 +117,9 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
 
     const uint64_t t_end = wall_clock64() + a.deadline;
     uint32_t ihead = 0;
-    int64_t n_upd = 0, n_vis = 0, n_idle = 0, n_fetch = 0, n_fetched = 0, n_exit = 0, n_res = 0;
+    int64_t n_upd = 0, n_vis = 0, n_idle = 0, n_fetch = 0, n_fetched = 0, n_exit = 0, n_res = 0, n_blocked = 0;
+    int64_t tk_visit = 0, tk_exit = 0, tk_fetch = 0, tk_chunk = 0;
+    int64_t tk_transit_l = 0;  // per lane
     bool hit_deadline = false;
     int idle = 0;
 
@@ -126,6 +128,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
         const uint32_t e = __builtin_amdgcn_readfirstlane(lds_ld(&sh.inbox[wave][ihead & (SLOTS - 1)]));
         if ((e >> 16) == (((ihead / SLOTS) + 1u) & 0xFFFFu)) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            const uint64_t t_v0 = wall_clock64();
             ++ihead;
             idle = 0;
             const int s = int(e & 0xFFFFu);
@@ -158,7 +161,9 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                 }
                 const bool from_hbm = __ballot(lane < left && !inwin) != 0;
                 const int32_t cnt = __popcll(__ballot(myrow < row_hi));
+                const uint64_t t_c0 = wall_clock64();
                 if (cnt > 0) run_chunk<S>(a, h, myrow, myval, cnt, step, reg, sr_b);
+                tk_chunk += int64_t(wall_clock64() - t_c0);
                 if (from_hbm) {  // keep this chunk for the waves that follow
                     if (lane < left) {
                         sh.wrow[s][lane] = myrow;
@@ -179,6 +184,8 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                 }
             }
             ++n_vis;
+            const uint64_t t_v1 = wall_clock64();
+            tk_visit += int64_t(t_v1 - t_v0);
             if (next_row >= 0) {
                 // more of this column lives in a later sub-block of this workgroup
                 sh.h[s][2 * lane] = h.x;
@@ -202,12 +209,14 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                     if (lane == 0) __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 } else {
                     const int32_t q = sh.nwg[s];
-                    uint32_t t = 0;
-                    if (lane == 0)
-                        t = __hip_atomic_fetch_add(a.qtail + q, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    // h_j and TokInfo complete before the slot is published
+                    // h_j and TokInfo complete before a ring position is even
+                    // reserved, so the consumer never waits long on a reserved
+                    // but unpublished entry
+                    if (lane == 0) st_dc(a.tok_stamp + j, uint64_t(wall_clock64()));
                     wait_all_mem();
                     if (lane == 0) {
+                        const uint32_t t =
+                            __hip_atomic_fetch_add(a.qtail + q, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         const uint64_t v = (uint64_t((t >> a.qlog2) + 1u) << 32) | uint32_t(j);
                         st_dc(a.qslot + (int64_t(q) << a.qlog2) + (t & gmask), v);
                     }
@@ -216,6 +225,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                     __hip_atomic_fetch_or(&sh.freebits[s >> 5], 1u << (s & 31), __ATOMIC_RELEASE,
                                           __HIP_MEMORY_SCOPE_WORKGROUP);
                 ++n_exit;
+                tk_exit += int64_t(wall_clock64() - t_v1);
             }
             if (wall_clock64() > t_end) {
                 hit_deadline = true;
@@ -241,6 +251,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
             const bool valid = uint32_t(sv >> 32) == (((gh + uint32_t(lane)) >> a.qlog2) + 1u);
             const uint64_t vmask = __ballot(valid);
             nb = vmask == ~0ull ? kWave : int(__builtin_ctzll(~vmask));
+            if (nb < kWave && (vmask >> nb) != 0) ++n_blocked;  // published entries behind an unpublished one
             int nfree = 0;
 #pragma unroll
             for (int w = 0; w < SLOTS / 32; ++w) nfree += __popc(lds_ld(&sh.freebits[w]));
@@ -270,6 +281,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
             }
         }
         if (nb > 0) {
+            const uint64_t t_f0 = wall_clock64();
             idle = 0;
             n_fetched += nb;
             // lane t < nb: metadata of claimed column t
@@ -293,6 +305,8 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                 nwg = nd.worker;
                 nstart = uint32_t(nd.start);
                 nlen = nd.len;
+                const uint64_t stamp = ld_dc(a.tok_stamp + jj);
+                if (stamp != 0) tk_transit_l += int64_t(wall_clock64() - stamp);
             }
             for (int t0 = 0; t0 < nb; t0 += 4) {
                 typename T::pair hv[4];
@@ -336,6 +350,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                     }
                 }
             }
+            tk_fetch += int64_t(wall_clock64() - t_f0);
             continue;
         }
         ++n_idle;
@@ -363,10 +378,13 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                                      uint32_t(j) * 0x01000193u + uint32_t(sh.cyc[s]), uint32_t(lane));
         T::store_dc(H + int64_t(j) * a.ldf, lane, npairs, h, rnd, a.round_mode);
     }
+    // lanes hold their own transit sums; fold them into lane 0
+    for (int off = 32; off > 0; off >>= 1) tk_transit_l += __shfl_down(tk_transit_l, off, 64);
     if (lane == 0) {
         int64_t* st = a.stats + int64_t(g) * nomad_k::kStatWidth;
-        const int64_t v[7] = {n_upd, n_vis, n_idle, n_fetch, n_fetched, n_exit, n_res};
-        for (int i = 0; i < 7; ++i)
+        const int64_t v[13] = {n_upd, n_vis, n_idle, n_fetch, n_fetched, n_exit, n_res, n_blocked,
+                               tk_visit, tk_exit, tk_fetch, tk_chunk, tk_transit_l};
+        for (int i = 0; i < 13; ++i)
             if (v[i]) __hip_atomic_fetch_add(st + i, v[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (hit_deadline) __hip_atomic_store(a.timed_out, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <numeric>
 #include <random>
@@ -85,6 +86,58 @@ void dev_free(T*& p) {
     p = nullptr;
 }
 
+// Memory that workgroups on different XCDs read and write during one kernel
+// (column rings, token records, h_j) must not sit in an XCD's L2 as a stale
+// copy: each XCD has its own L2 and plain (coarse-grained) device memory is
+// only made coherent at kernel boundaries.  Such buffers are therefore
+// allocated uncached, so every access reaches memory.  Uncached blocks are
+// taken from a process-wide pool and are never returned to the HIP allocator
+// while the process runs (freeing uncached memory that kernels have read has
+// corrupted later allocations on this platform); a destroyed context hands
+// its blocks back to the pool for reuse.
+struct CoherentBlock {
+    int device;
+    void* ptr;
+    size_t bytes;
+    bool used;
+};
+std::mutex g_coh_mu;
+std::vector<CoherentBlock> g_coh;
+
+int coh_alloc_bytes(int device, void** p, size_t bytes) {
+    *p = nullptr;
+    bytes = std::max<size_t>(4096, (bytes + 4095) & ~size_t(4095));
+    std::lock_guard<std::mutex> g(g_coh_mu);
+    CoherentBlock* best = nullptr;
+    for (auto& b : g_coh)
+        if (!b.used && b.device == device && b.bytes >= bytes && (!best || b.bytes < best->bytes)) best = &b;
+    if (best) {
+        best->used = true;
+        *p = best->ptr;
+        return NOMAD_OK;
+    }
+    hipError_t e = hipExtMallocWithFlags(p, bytes, hipDeviceMallocUncached);
+    if (e != hipSuccess)
+        return fail(NOMAD_ENOMEM, std::string("uncached hipExtMallocWithFlags(") + std::to_string(bytes) +
+                                      " bytes): " + hipGetErrorString(e));
+    g_coh.push_back(CoherentBlock{device, *p, bytes, true});
+    return NOMAD_OK;
+}
+
+template <class T>
+int coh_alloc(int device, T** p, size_t count) {
+    return coh_alloc_bytes(device, reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T));
+}
+
+template <class T>
+void coh_free(T*& p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(g_coh_mu);
+    for (auto& b : g_coh)
+        if (b.ptr == static_cast<void*>(p)) b.used = false;
+    p = nullptr;
+}
+
 struct DevCsc {
     int64_t nnz = -1;  // -1 = not uploaded
     int64_t* colptr = nullptr;
@@ -111,6 +164,7 @@ struct nomad_ctx {
 
     void* dW = nullptr;
     void* dH = nullptr;
+    bool h_coherent = false;  // dH lives in the uncached pool (async mode)
     DevCsc tr, te;
     std::vector<int64_t> h_colptr;  // training CSC kept on the host for routes
     std::vector<int32_t> h_rowidx;
@@ -128,6 +182,7 @@ struct nomad_ctx {
     int64_t* d_vptr = nullptr;
     VisitDesc* d_visits = nullptr;
     TokInfo* d_tok = nullptr;
+    uint64_t* d_tok_stamp = nullptr;
     uint64_t* d_qslot = nullptr;
     uint32_t* d_qtail = nullptr;
     uint32_t* d_qhead = nullptr;
@@ -146,7 +201,10 @@ struct nomad_ctx {
 
     ~nomad_ctx() {
         dev_free(dW);
-        dev_free(dH);
+        if (h_coherent)
+            coh_free(dH);
+        else
+            dev_free(dH);
         for (DevCsc* c : {&tr, &te}) {
             dev_free(c->colptr);
             dev_free(c->rowidx);
@@ -156,12 +214,13 @@ struct nomad_ctx {
         dev_free(d_partial);
         dev_free(d_vptr);
         dev_free(d_visits);
-        dev_free(d_tok);
-        dev_free(d_qslot);
-        dev_free(d_qtail);
-        dev_free(d_qhead);
-        dev_free(d_done);
-        dev_free(d_timed_out);
+        coh_free(d_tok);
+        coh_free(d_tok_stamp);
+        coh_free(d_qslot);
+        coh_free(d_qtail);
+        coh_free(d_qhead);
+        coh_free(d_done);
+        coh_free(d_timed_out);
         dev_free(d_stats);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -307,21 +366,24 @@ int setup_async(nomad_ctx* c) {
             return fail(NOMAD_EINVAL, "async mode: more than 2^31 visits or 2^32 ratings");
         dev_free(c->d_vptr);
         dev_free(c->d_visits);
-        dev_free(c->d_tok);
-        dev_free(c->d_qslot);
-        dev_free(c->d_qtail);
-        dev_free(c->d_qhead);
-        dev_free(c->d_done);
-        dev_free(c->d_timed_out);
+        coh_free(c->d_tok);
+        coh_free(c->d_tok_stamp);
+        coh_free(c->d_qslot);
+        coh_free(c->d_qtail);
+        coh_free(c->d_qhead);
+        coh_free(c->d_done);
+        coh_free(c->d_timed_out);
         dev_free(c->d_stats);
         if ((rc = dev_alloc(&c->d_vptr, size_t(n) + 1))) return rc;
         if ((rc = dev_alloc(&c->d_visits, size_t(c->V)))) return rc;
-        if ((rc = dev_alloc(&c->d_tok, size_t(n)))) return rc;
-        if ((rc = dev_alloc(&c->d_qslot, size_t(P) << c->qlog2))) return rc;
-        if ((rc = dev_alloc(&c->d_qtail, size_t(P)))) return rc;
-        if ((rc = dev_alloc(&c->d_qhead, size_t(P)))) return rc;
-        if ((rc = dev_alloc(&c->d_done, 1))) return rc;
-        if ((rc = dev_alloc(&c->d_timed_out, 1))) return rc;
+        if ((rc = coh_alloc(c->p.device, &c->d_tok, size_t(n)))) return rc;
+        if ((rc = coh_alloc(c->p.device, &c->d_tok_stamp, size_t(n)))) return rc;
+        NOMAD_HIP_TRY(hipMemset(c->d_tok_stamp, 0, size_t(n) * 8));
+        if ((rc = coh_alloc(c->p.device, &c->d_qslot, size_t(P) << c->qlog2))) return rc;
+        if ((rc = coh_alloc(c->p.device, &c->d_qtail, size_t(P)))) return rc;
+        if ((rc = coh_alloc(c->p.device, &c->d_qhead, size_t(P)))) return rc;
+        if ((rc = coh_alloc(c->p.device, &c->d_done, 1))) return rc;
+        if ((rc = coh_alloc(c->p.device, &c->d_timed_out, 1))) return rc;
         if ((rc = dev_alloc(&c->d_stats, size_t(P) * kStatWidth))) return rc;
         NOMAD_HIP_TRY(hipMemset(c->d_qslot, 0, (size_t(P) << c->qlog2) * 8));
         NOMAD_HIP_TRY(hipMemset(c->d_qtail, 0, size_t(P) * 4));
@@ -453,6 +515,7 @@ int train_async(nomad_ctx* c, double seconds, int64_t epochs, nomad_train_stats*
     a.vptr = c->d_vptr;
     a.visits = c->d_visits;
     a.tok = c->d_tok;
+    a.tok_stamp = c->d_tok_stamp;
     a.qslot = c->d_qslot;
     a.qtail = c->d_qtail;
     a.qhead = c->d_qhead;
@@ -679,7 +742,13 @@ int nomad_create(const nomad_params* p, nomad_ctx** out) {
 
     int rc;
     if ((rc = dev_alloc(reinterpret_cast<char**>(&c->dW), size_t(p->m) * c->ldf * c->esz))) return bail(rc);
-    if ((rc = dev_alloc(reinterpret_cast<char**>(&c->dH), size_t(p->n) * c->ldf * c->esz))) return bail(rc);
+    if (p->mode == NOMAD_MODE_ASYNC) {
+        c->h_coherent = true;
+        if ((rc = coh_alloc(p->device, reinterpret_cast<char**>(&c->dH), size_t(p->n) * c->ldf * c->esz)))
+            return bail(rc);
+    } else if ((rc = dev_alloc(reinterpret_cast<char**>(&c->dH), size_t(p->n) * c->ldf * c->esz))) {
+        return bail(rc);
+    }
     if (hipMemset(c->dW, 0, size_t(p->m) * c->ldf * c->esz) != hipSuccess ||
         hipMemset(c->dH, 0, size_t(p->n) * c->ldf * c->esz) != hipSuccess)
         return bail(fail(NOMAD_EHIP, "nomad_create: memset failed"));

@@ -26,7 +26,8 @@ struct alignas(16) TokInfo {
 
 // per-workgroup counters of the async kernel
 enum AsyncStat : int { kStatUpdates = 0, kStatVisits, kStatIdle, kStatFetches, kStatFetched, kStatExits,
-                       kStatResident, kStatWidth = 8 };
+                       kStatResident, kStatBlocked, kStatTicksVisit, kStatTicksExit, kStatTicksFetch,
+                       kStatTicksChunk, kStatTicksTransit, kStatWidth = 16 };
 
 struct SgdArgs {
     // problem
@@ -50,6 +51,7 @@ struct SgdArgs {
     const int64_t* vptr;       // n + 1 offsets into visits
     const VisitDesc* visits;   // per column, in route order; .worker = workgroup
     TokInfo* tok;              // per column: its next visit and completed cycles
+    uint64_t* tok_stamp;       // per column: wall clock when it was last published (diagnostics)
     uint64_t* qslot;           // nblocks x (1 << qlog2) ring slots: (lap+1) << 32 | column
     uint32_t* qtail;           // per workgroup, producers' reservation counter
     uint32_t* qhead;           // per workgroup, consumer position (persisted across launches)

@@ -285,9 +285,10 @@ class Context:
         return v.value
 
     def counters(self) -> dict:
-        out = np.zeros(8, np.int64)
-        self._check(self._lib.nomad_get_counters(self._h, _ptr(out, C.c_int64), 8))
-        names = ["updates", "visits", "idle_polls", "claims", "claimed", "exits", "resident_sum", "reserved"]
+        out = np.zeros(16, np.int64)
+        self._check(self._lib.nomad_get_counters(self._h, _ptr(out, C.c_int64), 16))
+        names = ["updates", "visits", "idle_polls", "claims", "claimed", "exits", "resident_sum", "blocked",
+                 "ticks_visit", "ticks_exit", "ticks_claim", "ticks_chain", "ticks_transit", "r13", "r14", "r15"]
         return dict(zip(names, out.tolist()))
 
     def upload(self, train: Csc, test: Csc | None = None) -> None:
