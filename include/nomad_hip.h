@@ -49,6 +49,12 @@ extern "C" {
 #define NOMAD_ROUND_NEAREST 0
 #define NOMAD_ROUND_STOCHASTIC 1
 
+/* nomad_params.flags */
+#define NOMAD_FLAG_BALANCE 1 /* async: row blocks hold about nnz / P training
+                                ratings each instead of ceil(m / P) rows     */
+#define NOMAD_FLAG_TIMING 2  /* async: collect the wall-clock counters [8..13]
+                                of nomad_get_counters (slows the kernel)    */
+
 /* error codes */
 #define NOMAD_OK 0
 #define NOMAD_EINVAL -1   /* bad argument                           */
@@ -69,7 +75,7 @@ typedef struct nomad_params {
     int32_t ptoken;    /* tokens a worker may hold ahead of the one it is
                           processing (README --ptoken); 0 = default            */
     int32_t device;    /* HIP device ordinal                                     */
-    int32_t flags;     /* reserved, must be 0                                    */
+    int32_t flags;     /* NOMAD_FLAG_* bits; others must be 0                    */
     double lrate;      /* --lrate  */
     double drate;      /* --drate  */
     double reg;        /* --reg    */
@@ -137,9 +143,13 @@ int nomad_get_nworkers(nomad_ctx* ctx, int32_t* nworkers);
  * unpublished one, [8] wall-clock ticks spent in visits, [9] ticks spent
  * handing columns to the next workgroup, [10] ticks spent claiming columns,
  * [11] ticks inside the SGD chains, [12] ticks between a column's
- * publication on a ring and its claim, [13..15] reserved.  Ticks run at the
- * device's constant wall-clock rate.  n is the capacity of out (at most 16
- * are written). */
+ * publication on a ring and its claim, [13] ticks slots waited in wave
+ * inboxes, [14] sum over work items of the wave's inbox depth, [15] work
+ * items (streams of pieces, or single long pieces) a wave ran.  Counters
+ * [8..13] are collected only with NOMAD_FLAG_TIMING.  Ticks run at the
+ * device's constant wall-clock rate.  When n exceeds 16, out[16 ..]
+ * receives the same 16 counters for each workgroup in turn, as far as n
+ * allows.  n is the capacity of out. */
 int nomad_get_counters(nomad_ctx* ctx, int64_t* out, int32_t n);
 
 #ifdef __cplusplus

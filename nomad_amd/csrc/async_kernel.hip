@@ -5,25 +5,33 @@
 // another machine (Code/nomad/nb_main.cpp:491-565, nb_sendrecv.cpp:189-204).
 // Here a workgroup of NW waves is the machine and each wave a thread:
 //
-//   * the workgroup owns NW * rpb consecutive rows of W, wave w the rpb rows
-//     of its sub-block (the reference's per-part split, nb_data.cpp:83-89);
+//   * the workgroup owns NW consecutive row blocks of W, wave w the rows
+//     [wbound[g*NW+w], wbound[g*NW+w+1]) (the reference splits rows evenly,
+//     nb_data.cpp:83-89; the host may instead balance the blocks by ratings);
 //   * columns arrive on the workgroup's ring in HBM (multi-producer, one
 //     consumer position in LDS).  An idle wave claims a batch of them, copies
 //     each h_j into an LDS slot (FP32, or FP64 for double storage) together
-//     with the column's first 64 ratings of this row range, and hands the
-//     slot to the wave owning the column's first row here;
-//   * a wave pops a slot from its LDS inbox, runs the SGD steps of its piece
-//     of the column (nb_main.cpp:520-542, rows ascending), writes h back to
-//     the slot and hands it to the wave owning the column's next row;
-//   * after the last piece h_j returns to HBM and the column is pushed to the
-//     next workgroup of its route; after the last workgroup of the route the
-//     column's cycle count (the step-size counter) advances, and a column
-//     whose count reaches target_cycle is parked.
+//     with a window of the column's first 64 ratings in this row range and
+//     the offsets at which each wave's rows start inside that window, and
+//     hands the slot to the wave owning the column's first row here;
+//   * a wave takes every slot waiting in its inbox whose piece (its rows of
+//     the column) lies inside the slot's window, up to 64 ratings in all, and
+//     runs their SGD steps as one stream (nb_main.cpp:520-542; rows ascending
+//     within a column, columns in inbox order).  Rows of W are prefetched 8-32
+//     ratings ahead across column boundaries; a row that occurs twice within
+//     that distance takes the value of its earlier update from registers.
+//     Slots whose piece is not inside the window are run one at a time,
+//     reading their ratings from HBM and refilling the window;
+//   * h goes back to the slot and the slot to the wave owning the column's
+//     next row; after the last piece h_j returns to HBM and the column is
+//     pushed to the next workgroup of its route; after the last workgroup of
+//     the route the column's cycle count (the step-size counter) advances,
+//     and a column whose count reaches target_cycle is parked.
 //
 // Cross-workgroup data (h_j, the column's TokInfo) moves with device-coherent
 // (sc1) loads and stores; a producer waits for all its memory operations to
-// complete before it publishes the ring slot, so no L2 writeback or
-// invalidate is needed.  W rows are private to their wave.
+// complete before it publishes ring slots, so no L2 writeback or invalidate
+// is needed.  W rows are private to their wave.
 #include <hip/hip_runtime.h>
 
 #include "sgd_common.hpp"
@@ -34,26 +42,49 @@ using nomad_k::TokInfo;
 namespace {
 
 constexpr int kFetchBatch = 16;  // columns claimed from the ring per fetch
+constexpr int kBatch = 16;       // inbox slots examined per stream
+// Updated W rows kept for forwarding, per wave, in LDS: a ring of as many
+// rows as the stream's prefetch distance, sized so that the rings of a
+// workgroup take 64 KiB.
+template <int S, int NW>
+constexpr int stream_depth() {
+    return (64 * 1024) / (NW * kWave * 2 * int(sizeof(typename Store<S>::acc)));
+}
+constexpr int kGroup = 8;  // ratings per unrolled group (stream depths are multiples)
+
+// W rows in flight inside one long piece: deep enough to cover HBM latency
+// with one wave per SIMD (4 waves per workgroup), less where more waves share
+// the SIMD's registers
+template <int NW>
+constexpr int long_prefetch_depth() {
+    return NW <= 4 ? 32 : NW <= 8 ? 16 : 8;
+}
 
 template <int S, int NW, int SLOTS>
 struct AsyncShared {
     using A = typename Store<S>::acc;
     using R = typename Store<S>::rval;
+    static constexpr int kPbw = (NW + 4) & ~3;
+    static constexpr int kVd = stream_depth<S, NW>();
+    A vic[NW][kVd][2 * kWave];   // per wave: the last kVd updated rows of its stream
     A h[SLOTS][2 * kWave];       // column factors of resident tokens
-    int32_t wrow[SLOTS][kWave];  // window of the visit's ratings: rows
-    R wval[SLOTS][kWave];        //                                values
+    int32_t wrow[SLOTS][kWave];  // window of the column's ratings here: rows
+    R wval[SLOTS][kWave];        //                                    values
     int32_t col[SLOTS];
-    uint32_t start[SLOTS];       // CSC offset of the visit in this row range
+    uint32_t start[SLOTS];       // CSC offset of the column's ratings in this row range
     int32_t len[SLOTS];
-    int32_t cur[SLOTS];          // ratings of the visit already processed
-    int32_t wb[SLOTS];           // the window holds visit offsets [wb, we)
+    int32_t cur[SLOTS];          // ratings of the range already processed
+    int32_t wb[SLOTS];           // the window holds range offsets [wb, we)
     int32_t we[SLOTS];
     int32_t cyc[SLOTS];          // completed route cycles of the column
+    A stp[SLOTS];                //   and the step size they give
     int32_t npos[SLOTS];         // route position of the column's next visit
     int32_t nwg[SLOTS];          //   its workgroup
     uint32_t nstart[SLOTS];      //   its CSC range
     int32_t nlen[SLOTS];
+    uint8_t pb[SLOTS][kPbw];     // pb[s][w]: window entries with a row below wave w's first row
     uint32_t inbox[NW][SLOTS];   // per wave: (lap << 16) | slot
+    uint32_t tpush[SLOTS];       // wall clock (low 32 bits) when the slot was last handed over
     uint32_t itail[NW];
     uint32_t freebits[SLOTS / 32];
     uint32_t lock;               // serialises ring claims and slot allocation
@@ -67,14 +98,44 @@ __device__ __forceinline__ uint32_t lds_ld(const uint32_t* p) {
 // Hand slot s to wave w of this workgroup.  Called by a whole wave; the
 // workgroup-scope release orders this wave's earlier LDS writes of the slot.
 template <int SLOTS, class L>
-__device__ __forceinline__ void inbox_push(L& sh, int w, int s) {
+__device__ __forceinline__ void inbox_push(L& sh, int w, int s, bool timing) {
     if (lane_id() == 0) {
+        if (timing) sh.tpush[s] = uint32_t(wall_clock64());
         const uint32_t t =
             __hip_atomic_fetch_add(&sh.itail[w], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         const uint32_t lap = ((t / uint32_t(SLOTS)) + 1u) & 0xFFFFu;
         __hip_atomic_store(&sh.inbox[w][t & uint32_t(SLOTS - 1)], (lap << 16) | uint32_t(s), __ATOMIC_RELEASE,
                            __HIP_MEMORY_SCOPE_WORKGROUP);
     }
+}
+
+// Wave of this workgroup that owns (wave-uniform) row r.
+__device__ __forceinline__ int wave_of(int32_t lane_bound, int32_t r) {
+    return __popcll(__ballot(r >= lane_bound));
+}
+
+// Same for a per-lane row.
+template <int NW>
+__device__ __forceinline__ int wave_of_lane(int32_t lane_bound, int32_t r) {
+    int w = 0;
+#pragma unroll
+    for (int b = 0; b < NW - 1; ++b) w += r >= readlane_i32(lane_bound, b) ? 1 : 0;
+    return w;
+}
+
+// Offsets of the waves' rows inside a freshly written window of slot s:
+// lane l holds the window's row l (valid for l < wlen).  pb[s][w] = number of
+// window rows below the first row of wave w.
+template <int NW, class L>
+__device__ __forceinline__ void write_bounds(L& sh, int s, int32_t myrow, int32_t wlen, int32_t lane_bound) {
+    const int lane = lane_id();
+    int32_t mine = lane == NW ? wlen : 0;
+#pragma unroll
+    for (int w = 1; w < NW; ++w) {
+        const int32_t c = __popcll(__ballot(lane < wlen && myrow < readlane_i32(lane_bound, w - 1)));
+        if (lane == w) mine = c;
+    }
+    if (lane <= NW) sh.pb[s][lane] = uint8_t(mine);
 }
 
 __device__ __forceinline__ void tok_store(TokInfo* p, uint32_t start, int32_t len, int32_t pos, int32_t cycle) {
@@ -91,103 +152,419 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
     using E = typename T::elem;
     using Aa = typename T::acc;
     using R = typename T::rval;
+    using P = typename T::pair;
     static_assert(SLOTS % 32 == 0 && (SLOTS & (SLOTS - 1)) == 0, "SLOTS must be a power of two >= 32");
+    static_assert(NW <= 16 && kBatch <= kWave, "layout limits");
+    constexpr int kPf = stream_depth<S, NW>();
+    constexpr int kG = kPf < kGroup ? kPf : kGroup;
+    static_assert(kPf % kG == 0, "stream depth must be a multiple of the group");
     __shared__ AsyncShared<S, NW, SLOTS> sh;
     const int g = blockIdx.x;
-    const int wave = threadIdx.x >> 6;
+    // wave-uniform values are made scalar so that branches on them stay
+    // scalar (a branch the compiler must treat as divergent ends every
+    // outstanding load with a full wait)
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
     const int lane = lane_id();
     const int npairs = a.ldf >> 1;
-    const int32_t wg_row0 = g * NW * a.rpb;
-    const int32_t row_hi = min(a.m, wg_row0 + (wave + 1) * a.rpb);
+    const int64_t ldf = a.ldf;
+    const int32_t* wb_g = a.wbound + int64_t(g) * NW;
+    const int32_t row_lo = __builtin_amdgcn_readfirstlane(wb_g[wave]);
+    const int32_t row_hi = __builtin_amdgcn_readfirstlane(wb_g[wave + 1]);
+    // lane t < NW - 1 holds the first row of wave t + 1 of this workgroup
+    const int32_t lane_bound = lane < NW - 1 ? wb_g[lane + 1] : INT_MAX;
     const uint32_t gmask = (1u << a.qlog2) - 1u;
     const uint64_t* ring = a.qslot + (int64_t(g) << a.qlog2);
+    E* __restrict__ W = static_cast<E*>(a.W);
     E* __restrict__ H = static_cast<E*>(a.H);
     const int32_t* __restrict__ rowidx = a.rowidx;
     const R* __restrict__ val = static_cast<const R*>(a.val);
     const Aa reg = Aa(a.reg);
+    // This wave's rows of W through one buffer resource.  The hardware range
+    // check covers the per-lane offset (not the scalar one), so the row's byte
+    // offset is added to it: lanes past the row, any row outside the block and
+    // the kDrop offset fall outside the range (loads read zero, stores are
+    // dropped).
+    const int rowbytes = a.ldf * int(sizeof(E));
+    const __amdgpu_buffer_rsrc_t wrs = __builtin_amdgcn_make_buffer_rsrc(
+        W + int64_t(row_lo) * ldf, 0, (row_hi - row_lo) * rowbytes, 0x00020000);
+    // (unsigned arithmetic: offsets past 2^31 wrap by design)
+    const uint32_t wvoff = lane < npairs ? uint32_t(lane * T::kPairBytes) : 0x80000000u;
+    auto woff = [&](int32_t r) { return int(wvoff + uint32_t(r - row_lo) * uint32_t(rowbytes)); };
+    const int kDrop = int(wvoff + 0x7FFFFFFFu);  // every lane out of range (also kDrop - d * 1024, d < 64)
 
     for (int i = threadIdx.x; i < NW * SLOTS; i += blockDim.x) (&sh.inbox[0][0])[i] = 0u;
     if (threadIdx.x < NW) sh.itail[threadIdx.x] = 0u;
     if (threadIdx.x < SLOTS / 32) sh.freebits[threadIdx.x] = 0xFFFFFFFFu;
     if (threadIdx.x == 0) {
         sh.lock = 0u;
-        sh.ghead = a.qhead[g];
+        sh.ghead = a.qhead[int64_t(g) * nomad_k::kQStride];
     }
     __syncthreads();
 
+    // Per-phase wall-clock counters only when asked for: each clock read is a
+    // scalar memory round trip.  The deadline is checked every 64 streams or
+    // visits and every 8 idle polls.
+    const bool timing = a.timing != 0;
+    auto clk = [timing]() -> uint64_t { return timing ? wall_clock64() : 0; };
     const uint64_t t_end = wall_clock64() + a.deadline;
     uint32_t ihead = 0;
     int64_t n_upd = 0, n_vis = 0, n_idle = 0, n_fetch = 0, n_fetched = 0, n_exit = 0, n_res = 0, n_blocked = 0;
-    int64_t tk_visit = 0, tk_exit = 0, tk_fetch = 0, tk_chunk = 0;
-    int64_t tk_transit_l = 0;  // per lane
+    int64_t tk_visit = 0, tk_exit = 0, tk_fetch = 0, tk_chunk = 0, n_depth = 0, n_work = 0;
+    int64_t tk_transit_l = 0, tk_inbox_l = 0;  // per lane
     bool hit_deadline = false;
     int idle = 0;
 
     for (;;) {
-        // ---- a slot waiting in this wave's inbox -------------------------------
-        const uint32_t e = __builtin_amdgcn_readfirstlane(lds_ld(&sh.inbox[wave][ihead & (SLOTS - 1)]));
-        if ((e >> 16) == (((ihead / SLOTS) + 1u) & 0xFFFFu)) {
+        // ---- slots waiting in this wave's inbox -----------------------------
+        uint32_t pe = 0;
+        bool rdy = false;
+        if (lane < kBatch) {
+            pe = lds_ld(&sh.inbox[wave][(ihead + uint32_t(lane)) & (SLOTS - 1)]);
+            rdy = (pe >> 16) == ((((ihead + uint32_t(lane)) / SLOTS) + 1u) & 0xFFFFu);
+        }
+        const int nr = int(__builtin_ctzll(~__ballot(rdy)));  // lanes >= kBatch are never ready
+        if (nr > 0) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            const uint64_t t_v0 = wall_clock64();
-            ++ihead;
             idle = 0;
-            const int s = int(e & 0xFFFFu);
+            const uint64_t t_v0 = clk();
+            n_depth += int64_t(lds_ld(&sh.itail[wave]) - ihead);
+            // lane t < nr: the t-th waiting slot and its piece in the window
+            const int ts = int(pe & uint32_t(SLOTS - 1));
+            int32_t t_wb = 0, t_lo = 0, t_hi = 0, t_cnt = 0;
+            bool t_ok = false;
+            if (lane < nr) {
+                const int32_t cur = sh.cur[ts];
+                t_wb = sh.wb[ts];
+                const int32_t we = sh.we[ts];
+                const int32_t wlen = we - t_wb;
+                t_lo = cur - t_wb;
+                t_hi = sh.pb[ts][wave + 1];
+                t_ok = cur >= t_wb && cur < we && (t_hi < wlen || we == sh.len[ts]) && t_hi > t_lo;
+                t_cnt = t_ok ? t_hi - t_lo : 0;
+                if (timing) tk_inbox_l += int64_t(uint32_t(t_v0) - sh.tpush[ts]);
+            }
+            // the longest run of leading slots whose pieces fit one 64-rating stream
+            int nt = 0;
+            int32_t total = 0, t_pref = 0;
+            for (int t = 0; t < nr; ++t) {
+                const int32_t c = readlane_i32(t_cnt, t);
+                if (c == 0 || total + c > kWave) break;
+                if (lane == t) t_pref = total;
+                total += c;
+                ++nt;
+            }
+            if (nt > 0) {
+                // ============ stream of nt pieces, total <= 64 ratings ===========
+                ihead += uint32_t(nt);
+                n_vis += nt;
+                n_upd += total;
+                ++n_work;
+                // lane l < total: rating l of the stream, from piece my_t
+                int my_t = 0;
+                for (int t = 1; t < nt; ++t) my_t += lane >= readlane_i32(t_pref, t) ? 1 : 0;
+                const int ms = __shfl(ts, my_t);
+                const int32_t mo = __shfl(t_lo, my_t) + lane - __shfl(t_pref, my_t);
+                int32_t srow = INT_MAX;
+                R sval = R(0);
+                if (lane < total) {
+                    srow = sh.wrow[ms][mo];
+                    sval = sh.wval[ms][mo];
+                }
+                Aa t_step = Aa(0);
+                uint32_t t_srb = 0;
+                int32_t t_cyc = 0;
+                if (lane < nt) {
+                    t_cyc = sh.cyc[ts];
+                    t_step = sh.stp[ts];
+                    t_srb = uint32_t(sh.col[ts]) * 0x01000193u + uint32_t(t_cyc);
+                }
+                // distance back to the latest earlier rating of the same row
+                // inside the prefetch window (0: none).  Its prefetched copy
+                // predates that rating's update, so the row is taken from the
+                // ring of updated rows instead.
+                int32_t pdist = 0;
+#pragma unroll
+                for (int b = kPf - 1; b >= 1; --b) {
+                    const int32_t o = __shfl_up(srow, b);
+                    if (lane >= b && lane < total && o == srow) pdist = b;
+                }
+
+                const uint64_t t_c0 = clk();
+                P wp[kPf];
+                Aa* const vring = &sh.vic[wave][0][0];
+                // Each prologue load is preceded by a store that the range check
+                // drops, so the prologue issues memory operations in the same
+                // pattern as the loop and the compiler's wait counts stay as
+                // loose as the prefetch distance allows.
+#pragma unroll
+                for (int d = 0; d < kPf; ++d) {
+                    const int32_t r0 = readlane_i32(srow, min(d, total - 1));
+                    // (distinct offsets: stores to one offset would be merged away)
+                    T::store_b(wrs, kDrop - d * 1024, 0, P{Aa(0), Aa(0)}, 0u, a.round_mode);
+                    wp[d] = T::load_b(wrs, woff(r0));
+                }
+                int cur_t = 0;
+                int s_cur = readlane_i32(ts, 0);
+                P h, hn;
+                h.x = sh.h[s_cur][2 * lane];
+                h.y = sh.h[s_cur][2 * lane + 1];
+                hn = h;
+                if (nt > 1) {
+                    const int s1 = readlane_i32(ts, 1);
+                    hn.x = sh.h[s1][2 * lane];
+                    hn.y = sh.h[s1][2 * lane + 1];
+                }
+                Aa step = readlane_val(t_step, 0);
+                uint32_t srb = uint32_t(readlane_i32(int32_t(t_srb), 0));
+                // kPf ratings per iteration in groups of kGroup.  A group that
+                // starts past the stream leaves both loops at once (an exit
+                // edge, so the loop keeps exact wait counts); slots of the last
+                // group past the stream are idle ratings whose store falls
+                // outside its buffer range and whose h update is discarded.
+                for (int u0 = 0;; u0 += kPf) {
+#pragma unroll
+                    for (int gq = 0; gq < kPf / kG; ++gq) {
+                        if (u0 + gq * kG >= total) goto stream_done;
+#pragma unroll
+                        for (int q = 0; q < kG; ++q) {
+                            const int d = gq * kG + q;
+                            const int u = u0 + d;
+                            const bool real = u < total;
+                            const int uc = real ? u : total - 1;
+                            const int32_t i = readlane_i32(srow, uc);
+                            const int t = readlane_i32(my_t, uc);
+                            if (t != cur_t) {  // next piece: park h, take the next column's
+                                sh.h[s_cur][2 * lane] = h.x;
+                                sh.h[s_cur][2 * lane + 1] = h.y;
+                                h = hn;
+                                cur_t = t;
+                                s_cur = readlane_i32(ts, t);
+                                step = readlane_val(t_step, t);
+                                srb = uint32_t(readlane_i32(int32_t(t_srb), t));
+                                if (t + 1 < nt) {
+                                    const int sn = readlane_i32(ts, t + 1);
+                                    hn.x = sh.h[sn][2 * lane];
+                                    hn.y = sh.h[sn][2 * lane + 1];
+                                }
+                            }
+                            // a row updated pd ratings ago is read back from the ring;
+                            // it goes to registers of its own, as writing it over
+                            // wp[d] would first wait for wp[d]'s load
+                            const int pd = readlane_i32(pdist, uc);
+                            P fw = P{Aa(0), Aa(0)};
+                            if (pd != 0) {
+                                const Aa* src = vring + ((d - pd + kPf) % kPf) * (2 * kWave);
+                                fw.x = src[2 * lane];
+                                fw.y = src[2 * lane + 1];
+                            }
+                            P w;
+                            w.x = pd != 0 ? fw.x : wp[d].x;
+                            w.y = pd != 0 ? fw.y : wp[d].y;
+                            const P h0 = h;
+                            sgd_step<P, Aa>(w, h, Aa(readlane_val(sval, uc)), step, reg);
+                            h.x = real ? h.x : h0.x;
+                            h.y = real ? h.y : h0.y;
+                            const uint32_t rnd = sr_bits(a.sr_seed, uint32_t(i), srb, uint32_t(lane));
+                            T::store_b(wrs, real ? woff(i) : kDrop, 0, w, rnd, a.round_mode);
+                            {
+                                const P wr = T::rounded(w, rnd, a.round_mode);
+                                Aa* dst = vring + d * (2 * kWave);
+                                dst[2 * lane] = wr.x;
+                                dst[2 * lane + 1] = wr.y;
+                            }
+                            const int un = min(u + kPf, total - 1);
+                            wp[d] = T::load_b(wrs, woff(readlane_i32(srow, un)));
+                        }
+                    }
+                }
+            stream_done:
+                sh.h[s_cur][2 * lane] = h.x;
+                sh.h[s_cur][2 * lane + 1] = h.y;
+                tk_chunk += int64_t(clk() - t_c0);
+
+                // ---- hand each slot on: next wave here, or out of the workgroup
+                const int32_t ncur = t_wb + t_hi;
+                bool t_exit = false;
+                int t_nw = 0;
+                if (lane < nt) {
+                    t_exit = ncur == sh.len[ts];
+                    // an eligible piece that does not end the range ends inside the window
+                    if (!t_exit) t_nw = wave_of_lane<NW>(lane_bound, sh.wrow[ts][t_hi]);
+                    sh.cur[ts] = ncur;
+                    if (timing && !t_exit) sh.tpush[ts] = uint32_t(wall_clock64());
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                if (lane < nt && !t_exit) {
+                    const uint32_t p =
+                        __hip_atomic_fetch_add(&sh.itail[t_nw], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    const uint32_t lap = ((p / uint32_t(SLOTS)) + 1u) & 0xFFFFu;
+                    __hip_atomic_store(&sh.inbox[t_nw][p & uint32_t(SLOTS - 1)], (lap << 16) | uint32_t(ts),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                const uint64_t xmask = __ballot(t_exit);
+                const uint64_t t_v1 = clk();
+                tk_visit += int64_t(t_v1 - t_v0);
+                if (xmask != 0) {
+                    // columns leaving the workgroup: h_j and TokInfo to HBM, one
+                    // wait for all of them, then their ring entries
+                    int32_t t_j = 0, t_np = 0, t_ncyc = 0;
+                    bool t_park = false;
+                    if (t_exit) {
+                        t_j = sh.col[ts];
+                        t_np = sh.npos[ts];
+                        t_ncyc = t_cyc + (t_np == 0 ? 1 : 0);
+                        t_park = t_np == 0 && t_ncyc >= a.target_cycle;
+                    }
+                    for (uint64_t mm = xmask; mm != 0; mm &= mm - 1) {
+                        const int t = int(__builtin_ctzll(mm));
+                        const int s = readlane_i32(ts, t);
+                        const int32_t j = readlane_i32(t_j, t);
+                        P hx;
+                        hx.x = sh.h[s][2 * lane];
+                        hx.y = sh.h[s][2 * lane + 1];
+                        const uint32_t rnd = sr_bits(a.sr_seed, 0x80000000u | uint32_t(g),
+                                                     uint32_t(readlane_i32(int32_t(t_srb), t)), uint32_t(lane));
+                        T::store_dc(H + int64_t(j) * ldf, lane, npairs, hx, rnd, a.round_mode);
+                        if (lane == 0) {
+                            tok_store(a.tok + j, sh.nstart[s], sh.nlen[s], readlane_i32(t_np, t),
+                                      readlane_i32(t_ncyc, t));
+                            if (timing) st_dc(a.tok_stamp + j, uint64_t(wall_clock64()));
+                        }
+                    }
+                    wait_all_mem();
+                    if (t_exit) {
+                        if (t_park) {
+                            __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        } else {
+                            const int32_t q = sh.nwg[ts];
+                            const uint32_t p = __hip_atomic_fetch_add(a.qtail + int64_t(q) * nomad_k::kQStride, 1u,
+                                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            const uint64_t v = (uint64_t((p >> a.qlog2) + 1u) << 32) | uint32_t(t_j);
+                            st_dc(a.qslot + (int64_t(q) << a.qlog2) + (p & gmask), v);
+                        }
+                        __hip_atomic_fetch_or(&sh.freebits[ts >> 5], 1u << (ts & 31), __ATOMIC_RELEASE,
+                                              __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                    n_exit += __popcll(xmask);
+                    tk_exit += int64_t(clk() - t_v1);
+                }
+                if ((n_work & 63) == 0 && wall_clock64() > t_end) {
+                    hit_deadline = true;
+                    break;
+                }
+                continue;
+            }
+
+            // ============ one slot whose piece is not inside its window ==========
+            ++ihead;
+            ++n_work;
+            const int s = readlane_i32(ts, 0);
             const int32_t j = sh.col[s];
             const uint32_t vstart = sh.start[s];
             const int32_t vlen = sh.len[s];
             int32_t cur = sh.cur[s];
             int32_t wb = sh.wb[s], we = sh.we[s];
             const int32_t cycle = sh.cyc[s];
-            const Aa step = step_for<Aa>(a, cycle);
+            const Aa step = sh.stp[s];
             const uint32_t sr_b = uint32_t(j) * 0x01000193u + uint32_t(cycle);
-            typename T::pair h;
+            P h;
             h.x = sh.h[s][2 * lane];
             h.y = sh.h[s][2 * lane + 1];
-            int32_t next_row = -1;
-            for (;;) {  // this wave's piece of the column, up to 64 ratings at a time
-                const int32_t left = vlen - cur;
-                const int32_t idx = cur + lane;
-                const bool inwin = idx >= wb && idx < we;
-                int32_t myrow = INT_MAX;
-                R myval = R(0);
-                if (lane < left) {
-                    if (inwin) {
-                        myrow = sh.wrow[s][idx - wb];
-                        myval = sh.wval[s][idx - wb];
-                    } else {
-                        myrow = rowidx[int64_t(vstart) + idx];
-                        myval = val[int64_t(vstart) + idx];
+            // Stream through the column's ratings from cur while the rows are
+            // this wave's: two 64-rating chunks of the CSC range in registers,
+            // W rows kPfL ratings ahead across the chunk boundary.  Rows of
+            // one column are distinct, so no update needs forwarding.
+            constexpr int kPfL = long_prefetch_depth<NW>();
+            const int64_t cbase = int64_t(vstart);
+            int32_t c0 = cur;  // range offset of chunk A
+            int32_t rA = INT_MAX, rB = INT_MAX;
+            R vA = R(0), vB = R(0);
+            if (c0 + lane < vlen) {
+                rA = rowidx[cbase + c0 + lane];
+                vA = val[cbase + c0 + lane];
+            }
+            if (c0 + kWave + lane < vlen) {
+                rB = rowidx[cbase + c0 + kWave + lane];
+                vB = val[cbase + c0 + kWave + lane];
+            }
+            const uint64_t t_c0 = clk();
+            // rows past the piece (another wave's, or the end of the range) are
+            // never updated; their prefetch slots re-read the piece's first row
+            const int32_t r_first = readlane_i32(rA, 0);
+            P wp[kPfL];
+#pragma unroll
+            for (int d = 0; d < kPfL; ++d) {
+                int32_t rr = readlane_i32(rA, d);
+                rr = rr < row_hi ? rr : r_first;
+                T::store_b(wrs, kDrop - d * 1024, 0, P{Aa(0), Aa(0)}, 0u, a.round_mode);  // dropped: see the stream prologue
+                wp[d] = T::load_b(wrs, woff(rr));
+            }
+            // piece ratings in chunk A: rows ascend, so they are a prefix
+            int32_t cntA = __popcll(__ballot(rA < row_hi));
+            int32_t cnt = 0;
+            for (;;) {
+                // groups of kPfL; idle ratings past the piece as in the stream above
+                for (int u0 = 0; u0 < cntA; u0 += kPfL) {
+#pragma unroll
+                    for (int d = 0; d < kPfL; ++d) {
+                        const int uu = u0 + d;
+                        const bool real = uu < cntA;
+                        const int uc = real ? uu : cntA - 1;
+                        const int32_t i = readlane_i32(rA, uc);
+                        const P h0 = h;
+                        sgd_step<P, Aa>(wp[d], h, Aa(readlane_val(vA, uc)), step, reg);
+                        h.x = real ? h.x : h0.x;
+                        h.y = real ? h.y : h0.y;
+                        const uint32_t rnd = sr_bits(a.sr_seed, uint32_t(i), sr_b, uint32_t(lane));
+                        T::store_b(wrs, real ? woff(i) : kDrop, 0, wp[d], rnd, a.round_mode);
+                        const int tg = uu + kPfL;  // < 2 * kWave
+                        const int32_t ra = readlane_i32(rA, tg & (kWave - 1));
+                        const int32_t rb = readlane_i32(rB, tg & (kWave - 1));
+                        int32_t rn = tg < kWave ? ra : rb;
+                        rn = rn < row_hi ? rn : i;
+                        wp[d] = T::load_b(wrs, woff(rn));
                     }
                 }
-                const bool from_hbm = __ballot(lane < left && !inwin) != 0;
-                const int32_t cnt = __popcll(__ballot(myrow < row_hi));
-                const uint64_t t_c0 = wall_clock64();
-                if (cnt > 0) run_chunk<S>(a, h, myrow, myval, cnt, step, reg, sr_b);
-                tk_chunk += int64_t(wall_clock64() - t_c0);
-                if (from_hbm) {  // keep this chunk for the waves that follow
-                    if (lane < left) {
-                        sh.wrow[s][lane] = myrow;
-                        sh.wval[s][lane] = myval;
-                    }
-                    wb = cur;
-                    we = cur + min(kWave, left);
-                    // later reads of the window by other lanes of this wave see these writes
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                cnt += cntA;
+                if (cntA < kWave) break;
+                // the piece goes on in the next chunk
+                rA = rB;
+                vA = vB;
+                c0 += kWave;
+                rB = INT_MAX;
+                vB = R(0);
+                if (c0 + kWave + lane < vlen) {
+                    rB = rowidx[cbase + c0 + kWave + lane];
+                    vB = val[cbase + c0 + kWave + lane];
                 }
-                n_upd += cnt;
-                cur += cnt;
-                if (cnt < kWave || cur >= vlen) {
-                    if (cur < vlen) next_row = readlane_i32(myrow, cnt);
-                    break;
-                }
+                cntA = __popcll(__ballot(rA < row_hi));
+                if (cntA == 0) break;
+            }
+            // first rating past the piece: position cntA of chunk A
+            const int stop_u = cntA;
+            const int32_t stop_row = readlane_i32(rA, stop_u & (kWave - 1));
+            tk_chunk += int64_t(clk() - t_c0);
+            n_upd += cnt;
+            cur += cnt;
+            const int32_t next_row = cur < vlen ? stop_row : -1;
+            if (next_row >= 0) {
+                // window of the following 64 ratings for the waves that follow
+                const int src = stop_u + lane;  // < 128
+                const int32_t wr_a = __shfl(rA, src & (kWave - 1)), wr_b = __shfl(rB, src & (kWave - 1));
+                const R wv_a = __shfl(vA, src & (kWave - 1)), wv_b = __shfl(vB, src & (kWave - 1));
+                const int32_t myrow = src < kWave ? wr_a : wr_b;
+                const R myval = src < kWave ? wv_a : wv_b;
+                sh.wrow[s][lane] = myrow;
+                sh.wval[s][lane] = myval;
+                wb = cur;
+                we = cur + min(kWave, vlen - cur);
+                write_bounds<NW>(sh, s, myrow, we - wb, lane_bound);
             }
             ++n_vis;
-            const uint64_t t_v1 = wall_clock64();
+            const uint64_t t_v1 = clk();
             tk_visit += int64_t(t_v1 - t_v0);
             if (next_row >= 0) {
-                // more of this column lives in a later sub-block of this workgroup
+                // more of this column lives in a later row block of this workgroup
                 sh.h[s][2 * lane] = h.x;
                 sh.h[s][2 * lane + 1] = h.y;
                 if (lane == 0) {
@@ -195,14 +572,14 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                     sh.wb[s] = wb;
                     sh.we[s] = we;
                 }
-                inbox_push<SLOTS>(sh, (next_row - wg_row0) / a.rpb, s);
+                inbox_push<SLOTS>(sh, wave_of(lane_bound, next_row), s, timing);
             } else {
                 // the column leaves this workgroup
                 const int32_t np = sh.npos[s];
                 const int32_t ncyc = cycle + (np == 0 ? 1 : 0);
                 const bool park = np == 0 && ncyc >= a.target_cycle;
                 const uint32_t rnd = sr_bits(a.sr_seed, 0x80000000u | uint32_t(g), sr_b, uint32_t(lane));
-                T::store_dc(H + int64_t(j) * a.ldf, lane, npairs, h, rnd, a.round_mode);
+                T::store_dc(H + int64_t(j) * ldf, lane, npairs, h, rnd, a.round_mode);
                 if (lane == 0) tok_store(a.tok + j, sh.nstart[s], sh.nlen[s], np, ncyc);
                 if (park) {
                     wait_all_mem();
@@ -212,11 +589,11 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                     // h_j and TokInfo complete before a ring position is even
                     // reserved, so the consumer never waits long on a reserved
                     // but unpublished entry
-                    if (lane == 0) st_dc(a.tok_stamp + j, uint64_t(wall_clock64()));
+                    if (timing && lane == 0) st_dc(a.tok_stamp + j, uint64_t(wall_clock64()));
                     wait_all_mem();
                     if (lane == 0) {
-                        const uint32_t t =
-                            __hip_atomic_fetch_add(a.qtail + q, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        const uint32_t t = __hip_atomic_fetch_add(a.qtail + int64_t(q) * nomad_k::kQStride, 1u,
+                                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         const uint64_t v = (uint64_t((t >> a.qlog2) + 1u) << 32) | uint32_t(j);
                         st_dc(a.qslot + (int64_t(q) << a.qlog2) + (t & gmask), v);
                     }
@@ -225,9 +602,9 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                     __hip_atomic_fetch_or(&sh.freebits[s >> 5], 1u << (s & 31), __ATOMIC_RELEASE,
                                           __HIP_MEMORY_SCOPE_WORKGROUP);
                 ++n_exit;
-                tk_exit += int64_t(wall_clock64() - t_v1);
+                tk_exit += int64_t(clk() - t_v1);
             }
-            if (wall_clock64() > t_end) {
+            if ((n_work & 63) == 0 && wall_clock64() > t_end) {
                 hit_deadline = true;
                 break;
             }
@@ -247,11 +624,21 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
             // reads its result back.  (Plain reads of words another lane just
             // changed are data races the compiler may resolve per lane.)
             const uint32_t gh = __builtin_amdgcn_readfirstlane(lds_ld(&sh.ghead));
-            sv = ld_dc(ring + ((gh + uint32_t(lane)) & gmask));
-            const bool valid = uint32_t(sv >> 32) == (((gh + uint32_t(lane)) >> a.qlog2) + 1u);
-            const uint64_t vmask = __ballot(valid);
-            nb = vmask == ~0ull ? kWave : int(__builtin_ctzll(~vmask));
-            if (nb < kWave && (vmask >> nb) != 0) ++n_blocked;  // published entries behind an unpublished one
+            // lane 0 looks at the head entry; the batch is read only when that
+            // one is published
+            uint64_t hv = 0;
+            if (lane == 0) hv = ld_dc(ring + (gh & gmask));
+            hv = uniform64(hv);
+            if (uint32_t(hv >> 32) == ((gh >> a.qlog2) + 1u)) {
+                bool valid = false;
+                if (lane < kFetchBatch) {
+                    sv = ld_dc(ring + ((gh + uint32_t(lane)) & gmask));
+                    valid = uint32_t(sv >> 32) == (((gh + uint32_t(lane)) >> a.qlog2) + 1u);
+                }
+                const uint64_t vmask = __ballot(valid);
+                nb = int(__builtin_ctzll(~vmask));  // lanes >= kFetchBatch are never valid
+                if ((vmask >> nb) != 0) ++n_blocked;  // published entries behind an unpublished one
+            }
             int nfree = 0;
 #pragma unroll
             for (int w = 0; w < SLOTS / 32; ++w) nfree += __popc(lds_ld(&sh.freebits[w]));
@@ -281,13 +668,14 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
             }
         }
         if (nb > 0) {
-            const uint64_t t_f0 = wall_clock64();
+            const uint64_t t_f0 = clk();
             idle = 0;
             n_fetched += nb;
             // lane t < nb: metadata of claimed column t
             const int32_t jj = int32_t(uint32_t(sv));
             uint32_t tstart = 0;
             int32_t tlen = 1, tpos = 0, tcyc = 0, nwg = 0, nlen = 1;
+            Aa tstp = Aa(0);
             uint32_t nstart = 0;
             int32_t np = 0;
             if (lane < nb) {
@@ -298,6 +686,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                 tlen = int32_t(w0 >> 32);
                 tpos = int32_t(uint32_t(w1));
                 tcyc = int32_t(w1 >> 32);
+                tstp = step_for<Aa>(a, tcyc);
                 const int64_t vb = a.vptr[jj];
                 const int32_t nv = int32_t(a.vptr[jj + 1] - vb);
                 np = tpos + 1 == nv ? 0 : tpos + 1;
@@ -305,11 +694,13 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                 nwg = nd.worker;
                 nstart = uint32_t(nd.start);
                 nlen = nd.len;
-                const uint64_t stamp = ld_dc(a.tok_stamp + jj);
-                if (stamp != 0) tk_transit_l += int64_t(wall_clock64() - stamp);
+                if (timing) {
+                    const uint64_t stamp = ld_dc(a.tok_stamp + jj);
+                    if (stamp != 0) tk_transit_l += int64_t(wall_clock64() - stamp);
+                }
             }
             for (int t0 = 0; t0 < nb; t0 += 4) {
-                typename T::pair hv[4];
+                P hv[4];
                 int32_t wr[4];
                 R wv[4];
 #pragma unroll
@@ -318,7 +709,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                     const int32_t j = readlane_i32(jj, t);
                     const uint32_t st = uint32_t(readlane_i32(int32_t(tstart), t));
                     const int32_t ln = readlane_i32(tlen, t);
-                    hv[u] = T::load_dc(H + int64_t(j) * a.ldf, lane, npairs);
+                    hv[u] = T::load_dc(H + int64_t(j) * ldf, lane, npairs);
                     const int32_t o = min(lane, ln - 1);
                     wr[u] = rowidx[int64_t(st) + o];
                     wv[u] = val[int64_t(st) + o];
@@ -333,6 +724,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                         sh.h[s][2 * lane + 1] = hv[u].y;
                         sh.wrow[s][lane] = wr[u];
                         sh.wval[s][lane] = wv[u];
+                        write_bounds<NW>(sh, s, wr[u], min(kWave, ln), lane_bound);
                         if (lane == 0) {
                             sh.col[s] = readlane_i32(jj, t);
                             sh.start[s] = uint32_t(readlane_i32(int32_t(tstart), t));
@@ -341,21 +733,24 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                             sh.wb[s] = 0;
                             sh.we[s] = min(kWave, ln);
                             sh.cyc[s] = readlane_i32(tcyc, t);
+                            sh.stp[s] = readlane_val(tstp, t);
                             sh.npos[s] = readlane_i32(np, t);
                             sh.nwg[s] = readlane_i32(nwg, t);
                             sh.nstart[s] = uint32_t(readlane_i32(int32_t(nstart), t));
                             sh.nlen[s] = readlane_i32(nlen, t);
                         }
-                        inbox_push<SLOTS>(sh, (readlane_i32(wr[u], 0) - wg_row0) / a.rpb, s);
+                        inbox_push<SLOTS>(sh, wave_of(lane_bound, readlane_i32(wr[u], 0)), s, timing);
                     }
                 }
             }
-            tk_fetch += int64_t(wall_clock64() - t_f0);
+            tk_fetch += int64_t(clk() - t_f0);
             continue;
         }
         ++n_idle;
-        if (int32_t(ld_dc(a.done)) >= a.ncols_active) break;
-        if (wall_clock64() > t_end) {
+        // the shared completion counter is read only now and then: every idle
+        // wave polling one word would load a single memory channel
+        if ((idle & 15) == 15 && int32_t(ld_dc(a.done)) >= a.ncols_active) break;
+        if ((idle & 7) == 7 && wall_clock64() > t_end) {
             hit_deadline = true;
             break;
         }
@@ -371,24 +766,27 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
     for (int s = wave; s < SLOTS; s += NW) {
         if (lds_ld(&sh.freebits[s >> 5]) & (1u << (s & 31))) continue;
         const int32_t j = sh.col[s];
-        typename T::pair h;
+        P h;
         h.x = sh.h[s][2 * lane];
         h.y = sh.h[s][2 * lane + 1];
         const uint32_t rnd = sr_bits(a.sr_seed, 0x80000000u | uint32_t(g),
                                      uint32_t(j) * 0x01000193u + uint32_t(sh.cyc[s]), uint32_t(lane));
-        T::store_dc(H + int64_t(j) * a.ldf, lane, npairs, h, rnd, a.round_mode);
+        T::store_dc(H + int64_t(j) * ldf, lane, npairs, h, rnd, a.round_mode);
     }
-    // lanes hold their own transit sums; fold them into lane 0
-    for (int off = 32; off > 0; off >>= 1) tk_transit_l += __shfl_down(tk_transit_l, off, 64);
+    // lanes hold their own sums; fold them into lane 0
+    for (int off = 32; off > 0; off >>= 1) {
+        tk_transit_l += __shfl_down(tk_transit_l, off, 64);
+        tk_inbox_l += __shfl_down(tk_inbox_l, off, 64);
+    }
     if (lane == 0) {
         int64_t* st = a.stats + int64_t(g) * nomad_k::kStatWidth;
-        const int64_t v[13] = {n_upd, n_vis, n_idle, n_fetch, n_fetched, n_exit, n_res, n_blocked,
-                               tk_visit, tk_exit, tk_fetch, tk_chunk, tk_transit_l};
-        for (int i = 0; i < 13; ++i)
+        const int64_t v[16] = {n_upd, n_vis, n_idle, n_fetch, n_fetched, n_exit, n_res, n_blocked,
+                               tk_visit, tk_exit, tk_fetch, tk_chunk, tk_transit_l, tk_inbox_l, n_depth, n_work};
+        for (int i = 0; i < 16; ++i)
             if (v[i]) __hip_atomic_fetch_add(st + i, v[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (hit_deadline) __hip_atomic_store(a.timed_out, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    if (threadIdx.x == 0) a.qhead[g] = lds_ld(&sh.ghead);
+    if (threadIdx.x == 0) a.qhead[int64_t(g) * nomad_k::kQStride] = lds_ld(&sh.ghead);
 }
 
 // Appends per-workgroup column lists to the rings before a launch: ring g
@@ -398,13 +796,17 @@ __global__ void seed_rings_kernel(SgdArgs a, const int32_t* list, const int64_t*
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.nblocks) return;
     const uint32_t mask = (1u << a.qlog2) - 1u;
-    uint32_t t = a.qtail[p];
+    uint32_t t = a.qtail[int64_t(p) * nomad_k::kQStride];
     for (int64_t e = off[p]; e < off[p + 1]; ++e, ++t) {
         a.qslot[(int64_t(p) << a.qlog2) + (t & mask)] = (uint64_t((t >> a.qlog2) + 1u) << 32) | uint32_t(list[e]);
     }
-    a.qtail[p] = t;
+    a.qtail[int64_t(p) * nomad_k::kQStride] = t;
 }
 
-template __global__ void sgd_async_kernel<NOMAD_STORE_FP32, 16, 128>(SgdArgs);
-template __global__ void sgd_async_kernel<NOMAD_STORE_BF16, 16, 128>(SgdArgs);
-template __global__ void sgd_async_kernel<NOMAD_STORE_FP64, 16, 64>(SgdArgs);
+#define NOMAD_ASYNC_INST(NW)                                                      \
+    template __global__ void sgd_async_kernel<NOMAD_STORE_FP32, NW, 64>(SgdArgs); \
+    template __global__ void sgd_async_kernel<NOMAD_STORE_BF16, NW, 64>(SgdArgs); \
+    template __global__ void sgd_async_kernel<NOMAD_STORE_FP64, NW, 32>(SgdArgs);
+NOMAD_ASYNC_INST(4)
+NOMAD_ASYNC_INST(8)
+NOMAD_ASYNC_INST(16)

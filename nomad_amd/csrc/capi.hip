@@ -147,7 +147,8 @@ struct DevCsc {
 
 }  // namespace
 
-constexpr int kHierWaves = 16;  // waves (row sub-blocks) per workgroup of the async kernel
+// waves (row blocks) per workgroup of the async kernel: 4, 8 or 16
+constexpr int kWaveChoices[3] = {4, 8, 16};
 constexpr int kStatWidth = nomad_k::kStatWidth;
 
 struct nomad_ctx {
@@ -155,7 +156,10 @@ struct nomad_ctx {
     int P = 0;  // row blocks: workers (async) or strata (strat)
     int rpb = 0, cpb = 0, ldf = 0;
     int units = 0;        // async: workgroups (route units)
-    int unit_rows = 0;    // rows per workgroup
+    int nw = 16;          // async: waves (row blocks) per workgroup
+    std::vector<int32_t> h_wbound;  // async: first row of each row block, P + 1 entries
+    std::vector<int32_t> row_unit;  // async: workgroup owning each row
+    int32_t* d_wbound = nullptr;
     size_t esz = 4, rvsz = 4;
     uint32_t sr_seed = 0;
     hipStream_t stream = nullptr;
@@ -191,6 +195,7 @@ struct nomad_ctx {
     uint32_t* d_timed_out = nullptr;
     int64_t* d_stats = nullptr;
     std::vector<int64_t> stats_prev;
+    std::vector<int64_t> last_unit_stats;  // per workgroup, last call
     int64_t last_stats[nomad_k::kStatWidth] = {0};
     int32_t ncols_active = 0;
     std::mt19937_64 rng_route;   // initial permutations (nb_main.cpp:79)
@@ -214,6 +219,7 @@ struct nomad_ctx {
         dev_free(d_partial);
         dev_free(d_vptr);
         dev_free(d_visits);
+        dev_free(d_wbound);
         coh_free(d_tok);
         coh_free(d_tok_stamp);
         coh_free(d_qslot);
@@ -298,6 +304,38 @@ int upload_csc(nomad_ctx* c, DevCsc& d, const int64_t* colptr, const int32_t* ro
     return NOMAD_OK;
 }
 
+// Row blocks of the async workers.  Default: ceil(m / P) rows each, the
+// reference's split (Code/nomad/nb_data.cpp:83-89).  With NOMAD_FLAG_BALANCE
+// the blocks are contiguous row ranges holding about nnz / P training ratings
+// each (block b starts at the first row whose rating prefix sum reaches
+// b * nnz / P).
+void build_partition(nomad_ctx* c) {
+    const int32_t m = c->p.m;
+    const int P = c->P;
+    c->h_wbound.assign(size_t(P) + 1, m);
+    if (c->p.flags & NOMAD_FLAG_BALANCE) {
+        std::vector<int64_t> deg(size_t(m) + 1, 0);
+        for (int32_t r : c->h_rowidx) deg[size_t(r) + 1]++;
+        for (int32_t r = 0; r < m; ++r) deg[size_t(r) + 1] += deg[size_t(r)];
+        const int64_t nnz = deg[size_t(m)];
+        int32_t r = 0;
+        for (int b = 0; b <= P; ++b) {
+            // first row r with prefix(r) >= ceil(b * nnz / P)
+            const int64_t want = (int64_t(b) * nnz + P - 1) / P;
+            while (r < m && deg[size_t(r)] < want) ++r;
+            c->h_wbound[size_t(b)] = r;
+        }
+        c->h_wbound[0] = 0;
+        c->h_wbound[size_t(P)] = m;
+    } else {
+        for (int b = 0; b <= P; ++b) c->h_wbound[size_t(b)] = int32_t(std::min<int64_t>(int64_t(b) * c->rpb, m));
+    }
+    c->row_unit.assign(size_t(m), 0);
+    for (int g = 0; g < c->units; ++g)
+        for (int32_t r = c->h_wbound[size_t(g) * c->nw]; r < c->h_wbound[size_t(g + 1) * c->nw]; ++r)
+            c->row_unit[size_t(r)] = g;
+}
+
 // Visit lists: for column j, the non-empty (column, worker) pairs ordered by
 // the column's worker permutation.  perm is a std::shuffle of 0..P-1 drawn
 // from rng, one per column in column order, like ColumnData::set_perm
@@ -320,9 +358,9 @@ void build_routes(nomad_ctx* c, std::mt19937_64& rng) {
         if (first) {
             const int64_t c0 = c->h_colptr[j], c1 = c->h_colptr[j + 1];
             for (int64_t e = c0; e < c1;) {
-                const int32_t wk = c->h_rowidx[e] / c->unit_rows;
+                const int32_t wk = c->row_unit[c->h_rowidx[e]];
                 int64_t f = e;
-                while (f < c1 && c->h_rowidx[f] / c->unit_rows == wk) ++f;
+                while (f < c1 && c->row_unit[c->h_rowidx[e]] == c->row_unit[c->h_rowidx[f]]) ++f;
                 runs.push_back(VisitDesc{e, int32_t(f - e), wk});
                 e = f;
             }
@@ -350,6 +388,7 @@ int setup_async(nomad_ctx* c) {
     if (!c->routes_ready) {
         c->h_vptr.clear();
         c->h_visits.clear();
+        build_partition(c);
         build_routes(c, c->rng_route);
         // ring capacity: more than the number of columns that can ever be
         // queued at one workgroup (its distinct columns)
@@ -366,6 +405,7 @@ int setup_async(nomad_ctx* c) {
             return fail(NOMAD_EINVAL, "async mode: more than 2^31 visits or 2^32 ratings");
         dev_free(c->d_vptr);
         dev_free(c->d_visits);
+        dev_free(c->d_wbound);
         coh_free(c->d_tok);
         coh_free(c->d_tok_stamp);
         coh_free(c->d_qslot);
@@ -376,18 +416,20 @@ int setup_async(nomad_ctx* c) {
         dev_free(c->d_stats);
         if ((rc = dev_alloc(&c->d_vptr, size_t(n) + 1))) return rc;
         if ((rc = dev_alloc(&c->d_visits, size_t(c->V)))) return rc;
+        if ((rc = dev_alloc(&c->d_wbound, c->h_wbound.size()))) return rc;
+        NOMAD_HIP_TRY(hipMemcpy(c->d_wbound, c->h_wbound.data(), c->h_wbound.size() * 4, hipMemcpyHostToDevice));
         if ((rc = coh_alloc(c->p.device, &c->d_tok, size_t(n)))) return rc;
         if ((rc = coh_alloc(c->p.device, &c->d_tok_stamp, size_t(n)))) return rc;
         NOMAD_HIP_TRY(hipMemset(c->d_tok_stamp, 0, size_t(n) * 8));
         if ((rc = coh_alloc(c->p.device, &c->d_qslot, size_t(P) << c->qlog2))) return rc;
-        if ((rc = coh_alloc(c->p.device, &c->d_qtail, size_t(P)))) return rc;
-        if ((rc = coh_alloc(c->p.device, &c->d_qhead, size_t(P)))) return rc;
+        if ((rc = coh_alloc(c->p.device, &c->d_qtail, size_t(P) * nomad_k::kQStride))) return rc;
+        if ((rc = coh_alloc(c->p.device, &c->d_qhead, size_t(P) * nomad_k::kQStride))) return rc;
         if ((rc = coh_alloc(c->p.device, &c->d_done, 1))) return rc;
         if ((rc = coh_alloc(c->p.device, &c->d_timed_out, 1))) return rc;
         if ((rc = dev_alloc(&c->d_stats, size_t(P) * kStatWidth))) return rc;
         NOMAD_HIP_TRY(hipMemset(c->d_qslot, 0, (size_t(P) << c->qlog2) * 8));
-        NOMAD_HIP_TRY(hipMemset(c->d_qtail, 0, size_t(P) * 4));
-        NOMAD_HIP_TRY(hipMemset(c->d_qhead, 0, size_t(P) * 4));
+        NOMAD_HIP_TRY(hipMemset(c->d_qtail, 0, size_t(P) * nomad_k::kQStride * 4));
+        NOMAD_HIP_TRY(hipMemset(c->d_qhead, 0, size_t(P) * nomad_k::kQStride * 4));
         NOMAD_HIP_TRY(hipMemset(c->d_stats, 0, size_t(P) * kStatWidth * 8));
         c->stats_prev.assign(size_t(P) * kStatWidth, 0);
         for (auto& t : tok) t = TokInfo{0, 0, 0, 0};
@@ -398,7 +440,7 @@ int setup_async(nomad_ctx* c) {
         NOMAD_HIP_TRY(hipMemcpy(tok.data(), c->d_tok, size_t(n) * sizeof(TokInfo), hipMemcpyDeviceToHost));
         build_routes(c, c->rng_reperm);
         // drop whatever is still queued: head = tail on every ring
-        NOMAD_HIP_TRY(hipMemcpy(c->d_qhead, c->d_qtail, size_t(P) * 4, hipMemcpyDeviceToDevice));
+        NOMAD_HIP_TRY(hipMemcpy(c->d_qhead, c->d_qtail, size_t(P) * nomad_k::kQStride * 4, hipMemcpyDeviceToDevice));
     }
     for (int32_t j = 0; j < n; ++j) {
         if (c->h_vptr[j + 1] > c->h_vptr[j]) {
@@ -446,10 +488,30 @@ hipError_t launch_strat(const SgdArgs& a, hipStream_t st) {
 }
 
 template <int S>
-hipError_t launch_async(const SgdArgs& a, hipStream_t st) {
-    constexpr int slots = S == NOMAD_STORE_FP64 ? 64 : 128;
-    hipLaunchKernelGGL((sgd_async_kernel<S, kHierWaves, slots>), dim3(a.nblocks), dim3(kHierWaves * 64), 0, st, a);
+constexpr int async_slots() {
+    return S == NOMAD_STORE_FP64 ? 32 : 64;
+}
+
+template <int S>
+hipError_t launch_async(const SgdArgs& a, int nw, hipStream_t st) {
+    constexpr int slots = async_slots<S>();
+    switch (nw) {
+        case 4: hipLaunchKernelGGL((sgd_async_kernel<S, 4, slots>), dim3(a.nblocks), dim3(4 * 64), 0, st, a); break;
+        case 8: hipLaunchKernelGGL((sgd_async_kernel<S, 8, slots>), dim3(a.nblocks), dim3(8 * 64), 0, st, a); break;
+        default: hipLaunchKernelGGL((sgd_async_kernel<S, 16, slots>), dim3(a.nblocks), dim3(16 * 64), 0, st, a);
+    }
     return hipGetLastError();
+}
+
+template <int S>
+hipError_t async_occupancy(int nw, int* per_cu) {
+    constexpr int slots = async_slots<S>();
+    switch (nw) {
+        case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, sgd_async_kernel<S, 4, slots>, 4 * 64, 0);
+        case 8: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, sgd_async_kernel<S, 8, slots>, 8 * 64, 0);
+        default:
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, sgd_async_kernel<S, 16, slots>, 16 * 64, 0);
+    }
 }
 
 template <int S>
@@ -512,6 +574,7 @@ int train_async(nomad_ctx* c, double seconds, int64_t epochs, nomad_train_stats*
         return rc;
     }
     SgdArgs a = base_args(c);
+    a.wbound = c->d_wbound;
     a.vptr = c->d_vptr;
     a.visits = c->d_visits;
     a.tok = c->d_tok;
@@ -525,6 +588,7 @@ int train_async(nomad_ctx* c, double seconds, int64_t epochs, nomad_train_stats*
     a.qlog2 = c->qlog2;
     a.target_cycle = int32_t(std::min<int64_t>(target, INT_MAX));
     a.ncols_active = c->ncols_active;
+    a.timing = (c->p.flags & NOMAD_FLAG_TIMING) ? 1 : 0;
     const double limit = seconds > 0 ? seconds : watchdog_seconds();
     a.deadline = uint64_t(limit * c->wall_hz);
     hipError_t e = hipMemsetAsync(c->d_done, 0, 4, c->stream);
@@ -537,7 +601,8 @@ int train_async(nomad_ctx* c, double seconds, int64_t epochs, nomad_train_stats*
     }
     if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
     if (e == hipSuccess)
-        e = dispatch_storage(c->p.storage, [&](auto s) { return launch_async<decltype(s)::value>(a, c->stream); });
+        e = dispatch_storage(c->p.storage,
+                             [&](auto s) { return launch_async<decltype(s)::value>(a, c->nw, c->stream); });
     if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     cleanup();
@@ -552,6 +617,8 @@ int train_async(nomad_ctx* c, double seconds, int64_t epochs, nomad_train_stats*
     int64_t sum[kStatWidth] = {0};
     for (int32_t q = 0; q < P; ++q)
         for (int i = 0; i < kStatWidth; ++i) sum[i] += stats[size_t(q) * kStatWidth + i] - c->stats_prev[size_t(q) * kStatWidth + i];
+    c->last_unit_stats.resize(stats.size());
+    for (size_t i = 0; i < stats.size(); ++i) c->last_unit_stats[i] = stats[i] - c->stats_prev[i];
     c->stats_prev.swap(stats);
     for (int i = 0; i < kStatWidth; ++i) c->last_stats[i] = sum[i];
     const int64_t upd = sum[nomad_k::kStatUpdates], vis = sum[nomad_k::kStatVisits], fails = sum[nomad_k::kStatIdle];
@@ -682,8 +749,8 @@ int nomad_create(const nomad_params* p, nomad_ctx** out) {
         return fail(NOMAD_EINVAL, "nomad_create: unknown mode");
     if (p->rounding != NOMAD_ROUND_NEAREST && p->rounding != NOMAD_ROUND_STOCHASTIC)
         return fail(NOMAD_EINVAL, "nomad_create: unknown rounding");
-    if (p->nworkers < 0 || p->ptoken < 0 || p->flags != 0)
-        return fail(NOMAD_EINVAL, "nomad_create: negative nworkers/ptoken or nonzero flags");
+    if (p->nworkers < 0 || p->ptoken < 0 || (p->flags & ~(NOMAD_FLAG_BALANCE | NOMAD_FLAG_TIMING)) != 0)
+        return fail(NOMAD_EINVAL, "nomad_create: negative nworkers/ptoken or unknown flags");
     if (!(p->lrate > 0) || !(p->drate >= 0) || !(p->reg >= 0))
         return fail(NOMAD_EINVAL, "nomad_create: lrate must be > 0, drate and reg >= 0");
     int ndev = 0;
@@ -714,30 +781,35 @@ int nomad_create(const nomad_params* p, nomad_ctx** out) {
 
     int P = p->nworkers;
     if (p->mode == NOMAD_MODE_ASYNC) {
-        // workgroups of kHierWaves waves; all must be co-resident
+        // P workers = G workgroups of nw waves, all co-resident: nw is the
+        // smallest of 4, 8, 16 that spreads P over the CUs
         int cus = 0, per_cu = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess)
             return bail(fail(NOMAD_EHIP, "nomad_create: device query failed"));
+        if (P <= 0) P = 16 * cus;
+        int nw = kWaveChoices[2];
+        for (int choice : kWaveChoices)
+            if (int64_t(choice) * cus >= P) {
+                nw = choice;
+                break;
+            }
         hipError_t oe = dispatch_storage(p->storage, [&](auto t) {
-            constexpr int S = decltype(t)::value;
-            constexpr int slots = S == NOMAD_STORE_FP64 ? 64 : 128;
-            return hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sgd_async_kernel<S, kHierWaves, slots>,
-                                                                kHierWaves * 64, 0);
+            return async_occupancy<decltype(t)::value>(nw, &per_cu);
         });
         if (oe != hipSuccess || per_cu < 1) return bail(fail(NOMAD_EHIP, "nomad_create: occupancy query failed"));
         const int gmax = cus * per_cu;
-        int G = P > 0 ? (P + kHierWaves - 1) / kHierWaves : gmax;
+        int G = (P + nw - 1) / nw;
         G = std::max(1, std::min(G, gmax));
-        G = std::max(1, std::min(G, (p->m + kHierWaves - 1) / kHierWaves));
-        P = G * kHierWaves;
+        G = std::max(1, std::min(G, (p->m + nw - 1) / nw));
+        P = G * nw;
         c->units = G;
+        c->nw = nw;
     } else if (P == 0) {
         P = 64;
     }
     if (p->mode == NOMAD_MODE_STRAT) P = std::max(1, std::min(P, p->m));
     c->P = P;
     c->rpb = p->m / P + ((p->m % P) > 0 ? 1 : 0);  // nb_data.cpp:83
-    if (p->mode == NOMAD_MODE_ASYNC) c->unit_rows = c->rpb * kHierWaves;
     if (p->mode == NOMAD_MODE_STRAT) c->cpb = p->n / P + ((p->n % P) > 0 ? 1 : 0);
 
     int rc;
@@ -773,6 +845,8 @@ int nomad_get_nworkers(nomad_ctx* c, int32_t* nworkers) {
 int nomad_get_counters(nomad_ctx* c, int64_t* out, int32_t n) {
     if (!c || !out || n < 0) return fail(NOMAD_EINVAL, "nomad_get_counters: bad argument");
     for (int32_t i = 0; i < std::min<int32_t>(n, kStatWidth); ++i) out[i] = c->last_stats[i];
+    for (int64_t i = kStatWidth; i < n && size_t(i - kStatWidth) < c->last_unit_stats.size(); ++i)
+        out[i] = c->last_unit_stats[size_t(i - kStatWidth)];
     return NOMAD_OK;
 }
 

@@ -106,6 +106,18 @@ __device__ __forceinline__ uint32_t sr_bits(uint32_t seed, uint32_t a, uint32_t 
 // data-dependent select after it, so several rows can be in flight.
 __device__ __align__(16) static double g_zero_pairs[2 * kWave];  // zero-initialised, never written
 
+// Buffer resource over one factor row with `bytes` valid bytes (gfx9 raw
+// buffer: stride 0, dword data format).  Accesses past `bytes` are dropped
+// (stores) or read as zero (loads) by the hardware's range check, so lanes
+// beyond the row need neither a branch nor an address select, and the
+// compiler counts every load and store exactly when it places its waits.
+// The row address must be wave-uniform (it lives in scalar registers).
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void* row, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(row), 0, bytes, 0x00020000);
+}
+
 // Device-coherent accesses: relaxed agent-scope atomics, i.e. loads and
 // stores with the sc1 bit, which are coherent across the XCDs' separate L2s
 // without the L2 writeback / invalidate that an agent-scope fence costs.
@@ -140,15 +152,28 @@ struct Store<NOMAD_STORE_FP32> {
         float x, y;
     };
     static __device__ __forceinline__ pair load(const elem* row, int lane, int npairs) {
-        const float2* src = lane < npairs ? reinterpret_cast<const float2*>(row)
-                                          : reinterpret_cast<const float2*>(g_zero_pairs);
-        const float2 v = src[lane];
-        return pair{v.x, v.y};
+        const u32x2_t v = __builtin_amdgcn_raw_buffer_load_b64(row_rsrc(row, npairs * 8), lane * 8, 0, 0);
+        return pair{__uint_as_float(v.x), __uint_as_float(v.y)};
     }
     static __device__ __forceinline__ void store(elem* row, int lane, int npairs, pair p, uint32_t,
                                                  int) {
-        if (lane < npairs) reinterpret_cast<float2*>(row)[lane] = make_float2(p.x, p.y);
+        const u32x2_t v = {__float_as_uint(p.x), __float_as_uint(p.y)};
+        __builtin_amdgcn_raw_buffer_store_b64(v, row_rsrc(row, npairs * 8), lane * 8, 0, 0);
     }
+    // rows of a block through one buffer resource: byte offset voff (per
+    // lane, out of range for lanes past the row) + soff (row within block)
+    static constexpr int kPairBytes = 8;
+    static __device__ __forceinline__ pair load_b(__amdgpu_buffer_rsrc_t r, int voff, int soff = 0) {
+        const u32x2_t v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+        return pair{__uint_as_float(v.x), __uint_as_float(v.y)};
+    }
+    static __device__ __forceinline__ void store_b(__amdgpu_buffer_rsrc_t r, int voff, int soff, pair p, uint32_t,
+                                                   int) {
+        const u32x2_t v = {__float_as_uint(p.x), __float_as_uint(p.y)};
+        __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, 0);
+    }
+    // the value a later load of a row stored with store() returns
+    static __device__ __forceinline__ pair rounded(pair p, uint32_t, int) { return p; }
     // device-coherent (sc1) variants for rows that move between workers
     static __device__ __forceinline__ pair load_dc(const elem* row, int lane, int npairs) {
         const uint64_t* src = lane < npairs ? reinterpret_cast<const uint64_t*>(row)
@@ -173,14 +198,28 @@ struct Store<NOMAD_STORE_FP64> {
         double x, y;
     };
     static __device__ __forceinline__ pair load(const elem* row, int lane, int npairs) {
-        const double2* src = lane < npairs ? reinterpret_cast<const double2*>(row)
-                                           : reinterpret_cast<const double2*>(g_zero_pairs);
-        const double2 v = src[lane];
-        return pair{v.x, v.y};
+        const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(row_rsrc(row, npairs * 16), lane * 16, 0, 0);
+        return pair{__builtin_bit_cast(double, (uint64_t(v.y) << 32) | v.x),
+                    __builtin_bit_cast(double, (uint64_t(v.w) << 32) | v.z)};
     }
     static __device__ __forceinline__ void store(elem* row, int lane, int npairs, pair p, uint32_t,
                                                  int) {
-        if (lane < npairs) reinterpret_cast<double2*>(row)[lane] = make_double2(p.x, p.y);
+        const uint64_t a = __builtin_bit_cast(uint64_t, p.x), b = __builtin_bit_cast(uint64_t, p.y);
+        const u32x4_t v = {uint32_t(a), uint32_t(a >> 32), uint32_t(b), uint32_t(b >> 32)};
+        __builtin_amdgcn_raw_buffer_store_b128(v, row_rsrc(row, npairs * 16), lane * 16, 0, 0);
+    }
+    static __device__ __forceinline__ pair rounded(pair p, uint32_t, int) { return p; }
+    static constexpr int kPairBytes = 16;
+    static __device__ __forceinline__ pair load_b(__amdgpu_buffer_rsrc_t r, int voff, int soff = 0) {
+        const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+        return pair{__builtin_bit_cast(double, (uint64_t(v.y) << 32) | v.x),
+                    __builtin_bit_cast(double, (uint64_t(v.w) << 32) | v.z)};
+    }
+    static __device__ __forceinline__ void store_b(__amdgpu_buffer_rsrc_t r, int voff, int soff, pair p, uint32_t,
+                                                   int) {
+        const uint64_t a = __builtin_bit_cast(uint64_t, p.x), b = __builtin_bit_cast(uint64_t, p.y);
+        const u32x4_t v = {uint32_t(a), uint32_t(a >> 32), uint32_t(b), uint32_t(b >> 32)};
+        __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, soff, 0);
     }
     static __device__ __forceinline__ pair load_dc(const elem* row, int lane, int npairs) {
         const uint64_t* src = lane < npairs ? reinterpret_cast<const uint64_t*>(row)
@@ -208,9 +247,7 @@ struct Store<NOMAD_STORE_BF16> {
     };
     // element pair (2l, 2l+1) is one aligned 32-bit word
     static __device__ __forceinline__ pair load(const elem* row, int lane, int npairs) {
-        const uint32_t* src = lane < npairs ? reinterpret_cast<const uint32_t*>(row)
-                                            : reinterpret_cast<const uint32_t*>(g_zero_pairs);
-        const uint32_t u = src[lane];
+        const uint32_t u = __builtin_amdgcn_raw_buffer_load_b32(row_rsrc(row, npairs * 4), lane * 4, 0, 0);
         return pair{bf16_lo(u), bf16_hi(u)};
     }
     static __device__ __forceinline__ void store(elem* row, int lane, int npairs, pair p, uint32_t rnd,
@@ -223,7 +260,35 @@ struct Store<NOMAD_STORE_BF16> {
             lo = f32_to_bf16_rne(p.x);
             hi = f32_to_bf16_rne(p.y);
         }
-        if (lane < npairs) reinterpret_cast<uint32_t*>(row)[lane] = lo | (hi << 16);
+        __builtin_amdgcn_raw_buffer_store_b32(lo | (hi << 16), row_rsrc(row, npairs * 4), lane * 4, 0, 0);
+    }
+    static constexpr int kPairBytes = 4;
+    static __device__ __forceinline__ pair load_b(__amdgpu_buffer_rsrc_t r, int voff, int soff = 0) {
+        const uint32_t u = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+        return pair{bf16_lo(u), bf16_hi(u)};
+    }
+    static __device__ __forceinline__ void store_b(__amdgpu_buffer_rsrc_t r, int voff, int soff, pair p,
+                                                   uint32_t rnd, int round_mode) {
+        uint32_t lo, hi;
+        if (round_mode == NOMAD_ROUND_STOCHASTIC) {
+            lo = f32_to_bf16_sr(p.x, rnd);
+            hi = f32_to_bf16_sr(p.y, rnd >> 16);
+        } else {
+            lo = f32_to_bf16_rne(p.x);
+            hi = f32_to_bf16_rne(p.y);
+        }
+        __builtin_amdgcn_raw_buffer_store_b32(lo | (hi << 16), r, voff, soff, 0);
+    }
+    static __device__ __forceinline__ pair rounded(pair p, uint32_t rnd, int round_mode) {
+        uint32_t lo, hi;
+        if (round_mode == NOMAD_ROUND_STOCHASTIC) {
+            lo = f32_to_bf16_sr(p.x, rnd);
+            hi = f32_to_bf16_sr(p.y, rnd >> 16);
+        } else {
+            lo = f32_to_bf16_rne(p.x);
+            hi = f32_to_bf16_rne(p.y);
+        }
+        return pair{__uint_as_float(lo << 16), __uint_as_float(hi << 16)};
     }
     static __device__ __forceinline__ pair load_dc(const elem* row, int lane, int npairs) {
         const uint32_t* src = lane < npairs ? reinterpret_cast<const uint32_t*>(row)
@@ -245,6 +310,11 @@ struct Store<NOMAD_STORE_BF16> {
     }
 };
 
+// Fused multiply-add in the arithmetic type itself (a single rounding in
+// FP32 for float, in FP64 for double).
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // One SGD step on (w_i, h_j) given the rating r.  Per lane, elements x and y:
 //   p   = h.x*w.x ; p = fma(h.y, w.y, p) ; dot = tree(p) ; err = dot - r
 //   a   = fma(err, h, reg*w)     b = fma(err, w, reg*h)
@@ -254,24 +324,24 @@ struct Store<NOMAD_STORE_BF16> {
 template <class P, class A>
 __device__ __forceinline__ A sgd_step(P& w, P& h, A r, A step, A reg) {
     A p = h.x * w.x;
-    p = __builtin_fma(h.y, w.y, p);
+    p = fma_t(h.y, w.y, p);
     const A dot = wave_tree_sum(p);
     const A err = dot - r;
-    const A ax = __builtin_fma(err, h.x, reg * w.x);
-    const A ay = __builtin_fma(err, h.y, reg * w.y);
-    const A bx = __builtin_fma(err, w.x, reg * h.x);
-    const A by = __builtin_fma(err, w.y, reg * h.y);
-    w.x = __builtin_fma(-step, ax, w.x);
-    w.y = __builtin_fma(-step, ay, w.y);
-    h.x = __builtin_fma(-step, bx, h.x);
-    h.y = __builtin_fma(-step, by, h.y);
+    const A ax = fma_t(err, h.x, reg * w.x);
+    const A ay = fma_t(err, h.y, reg * w.y);
+    const A bx = fma_t(err, w.x, reg * h.x);
+    const A by = fma_t(err, w.y, reg * h.y);
+    w.x = fma_t(-step, ax, w.x);
+    w.y = fma_t(-step, ay, w.y);
+    h.x = fma_t(-step, bx, h.x);
+    h.y = fma_t(-step, by, h.y);
     return err;
 }
 
 template <class A>
 __device__ __forceinline__ A wave_dot(A hx, A hy, A wx, A wy) {
     A p = hx * wx;
-    p = __builtin_fma(hy, wy, p);
+    p = fma_t(hy, wy, p);
     return wave_tree_sum(p);
 }
 

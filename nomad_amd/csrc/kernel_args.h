@@ -24,10 +24,14 @@ struct alignas(16) TokInfo {
     int32_t cycle;
 };
 
+// uint32 words between consecutive workgroups' ring counters (qtail, qhead):
+// each counter sits in its own 128-byte line
+constexpr int kQStride = 32;
+
 // per-workgroup counters of the async kernel
 enum AsyncStat : int { kStatUpdates = 0, kStatVisits, kStatIdle, kStatFetches, kStatFetched, kStatExits,
                        kStatResident, kStatBlocked, kStatTicksVisit, kStatTicksExit, kStatTicksFetch,
-                       kStatTicksChunk, kStatTicksTransit, kStatWidth = 16 };
+                       kStatTicksChunk, kStatTicksTransit, kStatTicksInbox, kStatDepth, kStatWork, kStatWidth = 16 };
 
 struct SgdArgs {
     // problem
@@ -48,20 +52,21 @@ struct SgdArgs {
     double reg;
 
     // async schedule (two-level: workgroups of waves)
+    const int32_t* wbound;     // nblocks * waves + 1 first rows of the worker row blocks
     const int64_t* vptr;       // n + 1 offsets into visits
     const VisitDesc* visits;   // per column, in route order; .worker = workgroup
     TokInfo* tok;              // per column: its next visit and completed cycles
     uint64_t* tok_stamp;       // per column: wall clock when it was last published (diagnostics)
     uint64_t* qslot;           // nblocks x (1 << qlog2) ring slots: (lap+1) << 32 | column
-    uint32_t* qtail;           // per workgroup, producers' reservation counter
-    uint32_t* qhead;           // per workgroup, consumer position (persisted across launches)
+    uint32_t* qtail;           // per workgroup (stride kQStride), producers' reservation counter
+    uint32_t* qhead;           // per workgroup (stride kQStride), consumer position (kept across launches)
     uint32_t* done;            // columns parked after reaching target_cycle
     int64_t* stats;            // per workgroup: kStatWidth counters
     uint32_t* timed_out;       // set when a workgroup stopped on the deadline
     int32_t qlog2;
     int32_t target_cycle;      // park a column when its cycle count reaches this
     int32_t ncols_active;      // columns circulating in this launch
-    int32_t pad0;
+    int32_t timing;            // nonzero: collect the per-phase wall-clock counters
     uint64_t deadline;         // wall_clock64 ticks after its start at which a worker stops
 
     // strat schedule

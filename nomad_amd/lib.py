@@ -19,6 +19,8 @@ LIBDIR = Path(__file__).resolve().parent / "lib"
 STORE_FP32, STORE_BF16, STORE_FP64 = 0, 1, 2
 MODE_ASYNC, MODE_STRAT = 0, 1
 ROUND_NEAREST, ROUND_STOCHASTIC = 0, 1
+FLAG_BALANCE = 1
+FLAG_TIMING = 2
 STORAGE_NAMES = {"fp32": STORE_FP32, "bf16": STORE_BF16, "fp64": STORE_FP64}
 
 
@@ -242,7 +244,8 @@ class Context:
 
     def __init__(self, m: int, n: int, k: int, *, storage: str | int = "fp32", mode: str | int = "async",
                  rounding: str | int = "nearest", nworkers: int = 0, ptoken: int = 0, device: int = 0,
-                 lrate: float = 0.001, drate: float = 0.1, reg: float = 1.0, seed: int = 12345):
+                 lrate: float = 0.001, drate: float = 0.1, reg: float = 1.0, seed: int = 12345,
+                 balance: bool = False, timing: bool = False):
         self._lib = hip()
         p = Params()
         p.m, p.n, p.k = m, n, k
@@ -250,7 +253,8 @@ class Context:
         p.mode = {"async": MODE_ASYNC, "strat": MODE_STRAT}[mode] if isinstance(mode, str) else mode
         p.rounding = ({"nearest": ROUND_NEAREST, "stochastic": ROUND_STOCHASTIC}[rounding]
                       if isinstance(rounding, str) else rounding)
-        p.nworkers, p.ptoken, p.device, p.flags = nworkers, ptoken, device, 0
+        p.nworkers, p.ptoken, p.device = nworkers, ptoken, device
+        p.flags = (FLAG_BALANCE if balance else 0) | (FLAG_TIMING if timing else 0)
         p.lrate, p.drate, p.reg, p.seed = lrate, drate, reg, seed
         self.params = p
         self._h = C.c_void_p()
@@ -284,12 +288,21 @@ class Context:
         self._check(self._lib.nomad_get_nworkers(self._h, C.byref(v)))
         return v.value
 
+    COUNTER_NAMES = ["updates", "visits", "idle_polls", "claims", "claimed", "exits", "resident_sum", "blocked",
+                     "ticks_visit", "ticks_exit", "ticks_claim", "ticks_chain", "ticks_transit", "ticks_inbox",
+                     "depth_sum", "work_items"]
+
     def counters(self) -> dict:
+        """Device counters of the last async train call (totals)."""
         out = np.zeros(16, np.int64)
         self._check(self._lib.nomad_get_counters(self._h, _ptr(out, C.c_int64), 16))
-        names = ["updates", "visits", "idle_polls", "claims", "claimed", "exits", "resident_sum", "blocked",
-                 "ticks_visit", "ticks_exit", "ticks_claim", "ticks_chain", "ticks_transit", "r13", "r14", "r15"]
-        return dict(zip(names, out.tolist()))
+        return dict(zip(self.COUNTER_NAMES, out.tolist()))
+
+    def unit_counters(self, units: int) -> np.ndarray:
+        """The same counters per workgroup: array (units, 16)."""
+        out = np.zeros(16 * (units + 1), np.int64)
+        self._check(self._lib.nomad_get_counters(self._h, _ptr(out, C.c_int64), out.size))
+        return out[16:].reshape(units, 16)
 
     def upload(self, train: Csc, test: Csc | None = None) -> None:
         for a, fn in ((train, self._lib.nomad_upload_train), (test, self._lib.nomad_upload_test)):
