@@ -5,15 +5,19 @@ items, 99,072,112 train + 1,408,395 test ratings, Zipf row/column degrees,
 integer ratings 1-5 from a rank-50 truth -- k = 100, reg 0.05, lrate 0.012,
 drate 0.05, factors initialised with the reference's streams.
 
-One step = one epoch of the asynchronous token kernel (every rating updated
-once) with inputs resident in HBM.  The FP32-stored path runs first; its final
-test RMSE x 1.005 is the target.  The BF16-stored path (stochastic rounding,
-FP32 arithmetic) then runs the same number of epochs, and the time at which it
-first reaches the target is reported.  `value` is the BF16 path's updates/s.
-Evaluation time is excluded from training time, as in the reference
-(Code/nomad/nb_main.cpp:265-266).
+One step = one training segment of the asynchronous token kernel: the
+columns circulate for a fixed device time (--segment seconds) with inputs
+resident in HBM, then training stops and the test RMSE is evaluated.  This is
+the reference's timeout schedule (Code/nomad/nb_main.cpp:109-134: train until
+the next timeout, stop, evaluate), and updates/s is its u= count over training
+time, evaluation excluded (nb_main.cpp:265-266, 284).  The FP32-stored path
+runs first; its final test RMSE x 1.005 is the target.  The BF16-stored path
+(stochastic rounding, FP32 arithmetic) then runs the same schedule on a fresh
+context, and the training time at which it first reaches the target is
+reported.  `value` is the BF16 path's updates/s.  Warmup segments run on a
+separate context that is then discarded.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale S]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--segment S] [--scale F]
 
 Prints one JSON line.
 """
@@ -38,23 +42,29 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_path(lib, tr, te, W0, H0, *, storage, rounding, k, hyp, nworkers, warmup, steps, target=None):
+def make_ctx(lib, tr, te, W0, H0, *, storage, rounding, k, hyp, nworkers):
     ctx = lib.Context(tr.m, tr.n, k, storage=storage, rounding=rounding, mode="async", nworkers=nworkers,
-                      seed=12345, **hyp)
+                      seed=12345, balance=True, **hyp)
     ctx.upload(tr, te)
     ctx.set_factors(W0, H0)
-    for _ in range(warmup):
-        ctx.train(epochs=1)
-    curve = []
+    return ctx
+
+
+def run_path(lib, tr, te, W0, H0, *, storage, rounding, k, hyp, nworkers, warmup, steps, segment, target=None):
+    if warmup > 0:
+        ctx = make_ctx(lib, tr, te, W0, H0, storage=storage, rounding=rounding, k=k, hyp=hyp, nworkers=nworkers)
+        for _ in range(warmup):
+            ctx.train(seconds=segment)
+        ctx.close()
+    ctx = make_ctx(lib, tr, te, W0, H0, storage=storage, rounding=rounding, k=k, hyp=hyp, nworkers=nworkers)
+    curve, times = [], []
     upd = 0
-    dev_s = 0.0
-    wall_s = 0.0
-    alg = 0.0
+    dev_s = wall_s = alg = 0.0
     visits = 0
     t_hit = None
-    for ep in range(steps):
+    for step in range(steps):
         t0 = time.perf_counter()
-        st = ctx.train(epochs=1)
+        st = ctx.train(seconds=segment)
         wall_s += time.perf_counter() - t0
         dev_s += st.train_seconds
         upd += st.updates
@@ -63,13 +73,14 @@ def run_path(lib, tr, te, W0, H0, *, storage, rounding, k, hyp, nworkers, warmup
         ev = ctx.eval()
         rmse = float(np.sqrt(ev.test_sse / max(ev.test_count, 1)))
         curve.append(round(rmse, 6))
+        times.append(round(dev_s, 4))
         if target is not None and t_hit is None and rmse <= target:
             t_hit = dev_s
-        log(f"{storage}: epoch {warmup + ep + 1} updates={st.updates} dev={st.train_seconds * 1e3:.1f} ms "
-            f"test RMSE={rmse:.6f}")
+        log(f"{storage}: segment {step + 1} updates={st.updates} dev={st.train_seconds * 1e3:.1f} ms "
+            f"epochs(min over columns)={st.epochs_done} test RMSE={rmse:.6f}")
     nw = ctx.nworkers
     ctx.close()
-    return dict(updates=upd, dev_s=dev_s, wall_s=wall_s, alg_bytes=alg, visits=visits, curve=curve,
+    return dict(updates=upd, dev_s=dev_s, wall_s=wall_s, alg_bytes=alg, visits=visits, curve=curve, times=times,
                 t_hit=t_hit, nworkers=nw)
 
 
@@ -87,10 +98,11 @@ def cpu_baseline(tr, W0, H0, k, hyp, seconds):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10, help="timed training segments per path")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed segments on a throw-away context")
+    ap.add_argument("--segment", type=float, default=0.1, help="device seconds per segment")
     ap.add_argument("--scale", type=float, default=1.0)
-    ap.add_argument("--nworkers", type=int, default=0)
+    ap.add_argument("--nworkers", type=int, default=2048)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp32", "bf16"], default=None)
@@ -110,27 +122,20 @@ def main() -> int:
     tr, te = lib.synth(spec)
     log(f"generated {tr.m} x {tr.n}, train {tr.nnz}, test {te.nnz} in {time.perf_counter() - t0:.1f} s")
     W0, H0 = lib.init_factors(tr.m, tr.n, k, 4, 12345)
+    common = dict(k=k, hyp=hyp, nworkers=a.nworkers, warmup=a.warmup, steps=a.steps, segment=a.segment)
 
     res = {}
+    target = None
     if a.only in (None, "fp32"):
-        res["fp32"] = run_path(lib, tr, te, W0, H0, storage="fp32", rounding="nearest", k=k, hyp=hyp,
-                               nworkers=a.nworkers, warmup=a.warmup, steps=a.steps)
+        res["fp32"] = run_path(lib, tr, te, W0, H0, storage="fp32", rounding="nearest", **common)
         target = res["fp32"]["curve"][-1] * 1.005
-        for ep, r in enumerate(res["fp32"]["curve"]):
-            pass
-    else:
-        target = None
+        res["fp32"]["t_hit"] = next((t for t, r in zip(res["fp32"]["times"], res["fp32"]["curve"]) if r <= target),
+                                    None)
     if a.only in (None, "bf16"):
-        res["bf16"] = run_path(lib, tr, te, W0, H0, storage="bf16", rounding="stochastic", k=k, hyp=hyp,
-                               nworkers=a.nworkers, warmup=a.warmup, steps=a.steps, target=target)
-    # FP32 time-to-target on its own curve (first epoch within the target)
-    if "fp32" in res:
-        acc = 0.0
-        per = res["fp32"]["dev_s"] / a.steps
-        res["fp32"]["t_hit"] = next(((i + 1) * per for i, r in enumerate(res["fp32"]["curve"]) if r <= target), None)
+        res["bf16"] = run_path(lib, tr, te, W0, H0, storage="bf16", rounding="stochastic", target=target, **common)
 
-    head = res.get("bf16") or res["fp32"]
     head_name = "bf16" if "bf16" in res else "fp32"
+    head = res[head_name]
     value = head["updates"] / head["dev_s"]
     achieved = head["alg_bytes"] / head["dev_s"] / 1e9
     out = {
@@ -147,10 +152,10 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "netflix-shape synthetic, async NOMAD token kernel, one epoch per step",
+            "workload": f"netflix-shape synthetic, async NOMAD token kernel, {a.segment} s training segments",
             "m": tr.m, "n": tr.n, "nnz_train": tr.nnz, "nnz_test": te.nnz, "k": k, **hyp,
             "storage": head_name, "rounding": "stochastic" if head_name == "bf16" else "nearest",
-            "workers": head["nworkers"], "scale": a.scale,
+            "workers": head["nworkers"], "row_blocks": "balanced by ratings", "scale": a.scale,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -162,10 +167,11 @@ def main() -> int:
         "paths": {
             name: {
                 "updates_per_s": r["updates"] / r["dev_s"],
-                "epoch_ms": r["dev_s"] / a.steps * 1e3,
-                "wall_epoch_ms": r["wall_s"] / a.steps * 1e3,
+                "segment_ms": r["dev_s"] / a.steps * 1e3,
+                "wall_segment_ms": r["wall_s"] / a.steps * 1e3,
                 "alg_GBps": r["alg_bytes"] / r["dev_s"] / 1e9,
-                "visits_per_epoch": r["visits"] / a.steps,
+                "visits_per_segment": r["visits"] / a.steps,
+                "train_seconds": r["times"],
                 "test_rmse_curve": r["curve"],
                 "time_to_target_s": r["t_hit"],
             } for name, r in res.items()

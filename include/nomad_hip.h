@@ -70,7 +70,7 @@ typedef struct nomad_params {
     int32_t storage;   /* NOMAD_STORE_*                                          */
     int32_t mode;      /* NOMAD_MODE_*                                           */
     int32_t rounding;  /* NOMAD_ROUND_* (bf16 only)                              */
-    int32_t nworkers;  /* async: workers = row blocks (0: fill the GPU);
+    int32_t nworkers;  /* async: workers = row blocks (0: 8 per compute unit);
                           strat: strata B (0: one per co-resident wave)          */
     int32_t ptoken;    /* tokens a worker may hold ahead of the one it is
                           processing (README --ptoken); 0 = default            */

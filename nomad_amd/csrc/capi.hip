@@ -786,7 +786,7 @@ int nomad_create(const nomad_params* p, nomad_ctx** out) {
         int cus = 0, per_cu = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess)
             return bail(fail(NOMAD_EHIP, "nomad_create: device query failed"));
-        if (P <= 0) P = 16 * cus;
+        if (P <= 0) P = 8 * cus;  // default: two worker waves per SIMD
         int nw = kWaveChoices[2];
         for (int choice : kWaveChoices)
             if (int64_t(choice) * cus >= P) {
