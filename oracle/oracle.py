@@ -37,6 +37,8 @@ _SYMS = {
     "orc_sgd_async": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _i64p, _i32p, _f64p, _f64p, _f64p,
                                 C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64,
                                 C.POINTER(AsyncStats)]),
+    "orc_dat_part_info": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _i32p, _i32p, _i64p, _i32p, _i32p, _i64p]),
+    "orc_dat_part_read": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _i32p, _i32p, _f64p]),
     "orc_sse": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _i64p, _i32p, _f64p, _f64p, _f64p]),
     "orc_loss_grad": (C.c_double, [C.c_int, C.c_int, C.c_int, _i64p, _i32p, _f64p, _f64p, _f64p, C.c_double,
                                    _f64p, _f64p]),
@@ -67,6 +69,26 @@ def _p(a, t):
 def _csc(a):
     return (np.ascontiguousarray(a.colptr, np.int64), np.ascontiguousarray(a.rowidx, np.int32),
             np.ascontiguousarray(a.val, np.float64))
+
+
+def dat_part(path, part, nparts):
+    """Part `part` of `nparts` of a .dat file as the reference reads it
+    (Code/nomad/nb_data.cpp:19-176): (nrows, ncols, total_nnz, min_row,
+    col_offset[ncols+1], local row ids, values)."""
+    m, n, mn, lr = (C.c_int() for _ in range(4))
+    tot, ln = C.c_int64(), C.c_int64()
+    rc = lib().orc_dat_part_info(str(path).encode(), part, nparts, C.byref(m), C.byref(n), C.byref(tot),
+                                 C.byref(mn), C.byref(lr), C.byref(ln))
+    if rc:
+        raise RuntimeError(f"orc_dat_part_info failed ({rc})")
+    col = np.zeros(n.value + 1, np.int32)
+    rows = np.zeros(ln.value, np.int32)
+    vals = np.zeros(ln.value, np.float64)
+    rc = lib().orc_dat_part_read(str(path).encode(), part, nparts, _p(col, C.c_int32), _p(rows, C.c_int32),
+                                 _p(vals, C.c_double))
+    if rc:
+        raise RuntimeError(f"orc_dat_part_read failed ({rc})")
+    return m.value, n.value, tot.value, mn.value, col, rows, vals
 
 
 def init_factors(m, n, k, nthreads, seed):
