@@ -659,8 +659,24 @@ static double rank_one_update(const int64_t* ptr, const int32_t* idx, const doub
 }
 
 int orc_ccd(int m, int n, int k, const int64_t* colptr, const int32_t* rowidx, const double* val, double* W,
-            double* H, double lambda, int iters, int inner, double eps, double* obj) {
+            double* H, double lambda, int iters, int inner, double eps, double* obj, const int64_t* te_colptr,
+            const int32_t* te_rowidx, const double* te_val, double* rmse) {
     if (m <= 0 || n <= 0 || k <= 0 || iters < 0 || inner <= 0) return -1;
+    // test RMSE of the current factors (the reference keeps the test residual
+    // up to date with UpdateRating, ccdr1-mpi-omp.cpp:444-447, 517-519)
+    auto test_rmse = [&]() {
+        double s = 0;
+        for (int j = 0; j < n; ++j)
+            for (int64_t e = te_colptr[j]; e < te_colptr[j + 1]; ++e) {
+                double dot = 0;
+                for (int d = 0; d < k; ++d) dot += W[int64_t(te_rowidx[e]) * k + d] * H[int64_t(j) * k + d];
+                const double r = te_val[e] - dot;
+                s += r * r;
+            }
+        return std::sqrt(s / double(te_colptr[n]));
+    };
+    if (rmse)
+        for (int64_t q = 0; q < int64_t(iters) * k; ++q) rmse[q] = std::nan("");
     const int64_t nnz = colptr[n];
     // CSR copy with the CSC->CSR position map
     std::vector<int64_t> rowptr(size_t(m) + 1, 0);
@@ -694,10 +710,11 @@ int orc_ccd(int m, int n, int k, const int64_t* colptr, const int32_t* rowidx, c
         for (int j = 0; j < n; ++j) reg += H[int64_t(j) * k + t] * H[int64_t(j) * k + t] * nnz_col(j);
     }
     std::vector<double> u(m), v(n);
-    double fundec_max = 0;
     if (obj)
         for (int64_t q = 0; q < int64_t(iters) * k; ++q) obj[q] = std::nan("");
     for (int oiter = 1; oiter <= iters; ++oiter) {
+        // both restart with every outer iteration (ccdr1-mpi-omp.cpp:424-428)
+        double fundec_max = 0;
         int early_stop = 0;
         for (int t = 0; t < k; ++t) {
             if (early_stop >= 5) break;
@@ -738,6 +755,7 @@ int orc_ccd(int m, int n, int k, const int64_t* colptr, const int32_t* rowidx, c
             for (int i = 0; i < m; ++i) reg += u[i] * u[i] * nnz_row(i);
             for (int j = 0; j < n; ++j) reg += v[j] * v[j] * nnz_col(j);
             if (obj) obj[int64_t(oiter - 1) * k + t] = loss + reg * lambda;
+            if (rmse && te_colptr && te_colptr[n] > 0) rmse[int64_t(oiter - 1) * k + t] = test_rmse();
         }
     }
     return 0;

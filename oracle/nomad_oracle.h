@@ -94,9 +94,12 @@ double orc_loss_grad(int m, int n, int k, const int64_t* colptr, const int32_t* 
  * `inner` inner iterations per rank, early stop with `eps`; lambda is weighted
  * by the row/column nnz as in RankOneUpdate.  obj[(iter-1)*k + t] receives
  * the objective printed after rank t of outer iteration iter (NaN for ranks
- * skipped by the early stop); obj may be NULL. */
+ * skipped by the early stop); obj may be NULL.  With a test CSC (te_*),
+ * rmse[(iter-1)*k + t] receives the test RMSE at the same points; te_* and
+ * rmse may be NULL. */
 int orc_ccd(int m, int n, int k, const int64_t* colptr, const int32_t* rowidx, const double* val,
-            double* W, double* H, double lambda, int iters, int inner, double eps, double* obj);
+            double* W, double* H, double lambda, int iters, int inner, double eps, double* obj,
+            const int64_t* te_colptr, const int32_t* te_rowidx, const double* te_val, double* rmse);
 
 #ifdef __cplusplus
 }

@@ -43,7 +43,7 @@ _SYMS = {
     "orc_loss_grad": (C.c_double, [C.c_int, C.c_int, C.c_int, _i64p, _i32p, _f64p, _f64p, _f64p, C.c_double,
                                    _f64p, _f64p]),
     "orc_ccd": (C.c_int, [C.c_int, C.c_int, C.c_int, _i64p, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_int,
-                          C.c_int, C.c_double, _f64p]),
+                          C.c_int, C.c_double, _f64p, _i64p, _i32p, _f64p, _f64p]),
 }
 
 _lib = None
@@ -156,14 +156,22 @@ def loss_grad(a, W, H, reg, want_grad=True):
     return L, gW, gH
 
 
-def ccd(a, W, H, lam, iters, inner, eps=1e-3):
+def ccd(a, W, H, lam, iters, inner, eps=1e-3, test=None):
+    """CCD++ restatement; returns W, H, obj[iters*k] and, with a test CSC,
+    rmse[iters*k] (NaN where the early stop skipped a rank)."""
     colptr, rowidx, val = _csc(a)
     W = np.array(W, np.float64, copy=True, order="C")
     H = np.array(H, np.float64, copy=True, order="C")
     k = W.shape[1]
     obj = np.zeros(iters * k)
+    rmse = np.zeros(iters * k)
+    te = _csc(test) if test is not None else None
     rc = lib().orc_ccd(a.m, a.n, k, _p(colptr, C.c_int64), _p(rowidx, C.c_int32), _p(val, C.c_double),
-                       _p(W, C.c_double), _p(H, C.c_double), lam, iters, inner, eps, _p(obj, C.c_double))
+                       _p(W, C.c_double), _p(H, C.c_double), lam, iters, inner, eps, _p(obj, C.c_double),
+                       None if te is None else _p(te[0], C.c_int64), None if te is None else _p(te[1], C.c_int32),
+                       None if te is None else _p(te[2], C.c_double), None if te is None else _p(rmse, C.c_double))
     if rc != 0:
         raise RuntimeError("orc_ccd failed")
-    return W, H, obj
+    if test is None:
+        return W, H, obj
+    return W, H, obj, rmse

@@ -1,6 +1,9 @@
 """CPU checks that pin the oracle against things other than itself: the C++
 standard's MT19937-64 definition, closed forms, hand-computed updates, numpy
 FP64 replays, finite differences and invariants."""
+import json
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -230,3 +233,40 @@ def test_ccd_decreases_objective():
     _, _, obj = O.ccd(tr, W, H, 0.1, 3, 3)
     o = obj[~np.isnan(obj)]
     assert (np.diff(o) <= 1e-9 * o[:-1]).all()
+
+
+# ---- pinned against the reference competitor programs (tests/golden) --------
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def ref_problem():
+    z = np.load(GOLDEN / "ref_problem.npz")
+    m, n = int(z["m"]), int(z["n"])
+    tr = lib.Csc(m, n, z["tr_colptr"], z["tr_rowidx"], z["tr_val"])
+    te = lib.Csc(m, n, z["te_colptr"], z["te_rowidx"], z["te_val"])
+    return tr, te, json.loads((GOLDEN / "ref_traces.json").read_text())
+
+
+def test_objective_matches_reference_dsgd_print():
+    # sgd-mpi-omp with learning rate 0 prints calobj_mpi of its initial
+    # factors: W from mt19937_64(12345 + 1), H from mt19937_64(12345 + 139)
+    tr, _, g = ref_problem()
+    k = g["k"]
+    W, H = O.init_factors(tr.m, tr.n, k, 1, 12345)
+    L, _, _ = O.loss_grad(tr, W, H, g["dsgd"]["reg"], want_grad=False)
+    assert L == pytest.approx(g["dsgd"]["obj0"], rel=5e-6)
+
+
+def test_ccd_matches_reference_ccdr1_trace():
+    tr, te, g = ref_problem()
+    k, c = g["k"], g["ccd"]
+    W, _ = O.init_factors(tr.m, tr.n, k, 1, 12345)  # one thread: mt19937_64(12345 + 1)
+    H = np.zeros((tr.n, k))
+    _, _, obj, rmse = O.ccd(tr, W, H, c["lam"], c["iters"], c["inner"], c["eps"], test=te)
+    for p in c["trace"]:
+        q = (p["iter"] - 1) * k + p["rank"] - 1
+        assert obj[q] == pytest.approx(p["obj"], rel=6e-6), p
+        assert rmse[q] == pytest.approx(p["rmse"], rel=1e-9), p
+    # ranks the reference did not print were skipped by its early stop
+    printed = {(p["iter"] - 1) * k + p["rank"] - 1 for p in c["trace"]}
+    assert all(np.isnan(obj[q]) for q in range(len(obj)) if q not in printed)
