@@ -152,6 +152,65 @@ int nomad_get_nworkers(nomad_ctx* ctx, int32_t* nworkers);
  * allows.  n is the capacity of out. */
 int nomad_get_counters(nomad_ctx* ctx, int64_t* out, int32_t n);
 
+/* ---- CCD++ rank-one solver (the reference's competitor program) ----------
+ * Replaces ccdr1_mpi at one process (Code/competitor/ccdr1-mpi-omp.cpp:398-558:
+ * RankOneUpdate :334-358, UpdateRating :360-385, the rank / inner-iteration
+ * loop with its fundec early stop :430-530), in FP64 on the GPU.  W (m x k)
+ * and H (n x k) are row-major host arrays, read as the start point and
+ * overwritten with the result (the reference starts from H = 0).  obj and
+ * rmse (may be NULL) receive, at [(iter-1)*k + t], the objective
+ * loss + lambda * sum(nnz-weighted norms) and the test RMSE after rank t of
+ * outer iteration iter, NaN where the early stop skipped the rank.  The test
+ * CSC may be NULL / 0.  nnz < 2^31. */
+typedef struct nomad_ccd_params {
+    int32_t m, n, k;  /* rows, columns, rank (-k)              */
+    int32_t iters;    /* outer iterations (-t)                 */
+    int32_t inner;    /* inner iterations per rank (-T)        */
+    int32_t device;   /* HIP device ordinal                    */
+    double lambda;    /* -l; weighted by each line's nnz       */
+    double eps;       /* -e; inner stop fundec < max * eps     */
+} nomad_ccd_params;
+
+typedef struct nomad_ccd_stats {
+    int64_t ranks_done;  /* ranks updated (not skipped)             */
+    int64_t inner_done;  /* inner iterations run                    */
+    double seconds;      /* wall time of the call                   */
+    double alg_bytes;    /* algorithmic bytes: per inner iteration
+                            2 nnz (4 + 8) + 16 (m + n); per rank
+                            32 (nnz + test nnz) (residual updates)  */
+} nomad_ccd_stats;
+
+int nomad_ccd(const nomad_ccd_params* params, const int64_t* colptr, const int32_t* rowidx, const double* val,
+              int64_t nnz, const int64_t* te_colptr, const int32_t* te_rowidx, const double* te_val,
+              int64_t te_nnz, double* W, double* H, double* obj, double* rmse, nomad_ccd_stats* stats);
+
+/* ---- objective and gradient ---------------------------------------------
+ * L = sum (w_i.h_j - r_ij)^2 + reg (sum_i deg(i) |w_i|^2 + sum_j deg(j) |h_j|^2),
+ * the objective the reference's competitors print (calobj_mpi,
+ * Code/competitor/sgd-mpi-omp.cpp:122-164), and its gradient
+ * dL/dw_i = 2 sum_j e_ij h_j + 2 reg deg(i) w_i (likewise for h_j), e = w.h - r.
+ * No reference function computes the gradient; it is the closed form.
+ * nomad_obj_create copies the ratings (host CSC) to the device once.
+ * nomad_obj_eval reads W (m x ldf) and H (n x ldf), ldf = k rounded up to
+ * even, row-major DEVICE arrays of the context's storage type (padding
+ * columns zero), writes the gradients to device arrays of the same shapes,
+ * runs on `stream` (a hipStream_t, NULL: the null stream) and returns L on
+ * the host after synchronising that stream. */
+typedef struct nomad_obj_params {
+    int32_t m, n, k;
+    int32_t storage;  /* NOMAD_STORE_FP64 or NOMAD_STORE_FP32 (FP32 arithmetic,
+                         FP64 sums of squares)                                 */
+    int32_t device;
+} nomad_obj_params;
+
+typedef struct nomad_obj_ctx nomad_obj_ctx;
+
+int nomad_obj_create(const nomad_obj_params* params, const int64_t* colptr, const int32_t* rowidx,
+                     const double* val, int64_t nnz, nomad_obj_ctx** out);
+int nomad_obj_eval(nomad_obj_ctx* ctx, const void* W, const void* H, double reg, void* gW, void* gH,
+                   double* loss, void* stream);
+int nomad_obj_destroy(nomad_obj_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

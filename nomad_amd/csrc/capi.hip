@@ -151,6 +151,11 @@ struct DevCsc {
 constexpr int kWaveChoices[3] = {4, 8, 16};
 constexpr int kStatWidth = nomad_k::kStatWidth;
 
+// error reporting shared with the library's other translation units
+namespace nomad_host {
+int fail(int code, const std::string& msg) { return ::fail(code, msg); }
+}  // namespace nomad_host
+
 struct nomad_ctx {
     nomad_params p{};
     int P = 0;  // row blocks: workers (async) or strata (strat)

@@ -54,6 +54,23 @@ class EvalStats(C.Structure):
     ]
 
 
+class CcdParams(C.Structure):
+    _fields_ = [
+        ("m", C.c_int32), ("n", C.c_int32), ("k", C.c_int32), ("iters", C.c_int32), ("inner", C.c_int32),
+        ("device", C.c_int32), ("lam", C.c_double), ("eps", C.c_double),
+    ]
+
+
+class CcdStats(C.Structure):
+    _fields_ = [
+        ("ranks_done", C.c_int64), ("inner_done", C.c_int64), ("seconds", C.c_double), ("alg_bytes", C.c_double),
+    ]
+
+
+class ObjParams(C.Structure):
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("k", C.c_int32), ("storage", C.c_int32), ("device", C.c_int32)]
+
+
 class SynthSpec(C.Structure):
     _fields_ = [
         ("m", C.c_int32), ("n", C.c_int32), ("nnz_train", C.c_int64), ("nnz_test", C.c_int64),
@@ -83,6 +100,11 @@ HIP_SYMBOLS = {
     "nomad_eval": (C.c_int, [_P, C.POINTER(EvalStats)]),
     "nomad_get_nworkers": (C.c_int, [_P, _i32p]),
     "nomad_get_counters": (C.c_int, [_P, _i64p, C.c_int32]),
+    "nomad_obj_create": (C.c_int, [C.POINTER(ObjParams), _i64p, _i32p, _f64p, C.c_int64, C.POINTER(_P)]),
+    "nomad_obj_eval": (C.c_int, [_P, _P, _P, C.c_double, _P, _P, C.POINTER(C.c_double), _P]),
+    "nomad_obj_destroy": (C.c_int, [_P]),
+    "nomad_ccd": (C.c_int, [C.POINTER(CcdParams), _i64p, _i32p, _f64p, C.c_int64, _i64p, _i32p, _f64p, C.c_int64,
+                            _f64p, _f64p, _f64p, _f64p, C.POINTER(CcdStats)]),
 }
 
 TOOLS_SYMBOLS = {
@@ -236,6 +258,73 @@ def init_factors(m: int, n: int, k: int, nthreads: int, seed: int):
     _tools_check(tools().nomad_tools_init_factors(m, n, k, nthreads, seed, _ptr(W, C.c_double),
                                                   _ptr(H, C.c_double)))
     return W, H
+
+
+def ccd(train: Csc, W, H, lam: float, iters: int, inner: int, eps: float = 1e-3, test: Csc | None = None,
+        device: int = 0):
+    """CCD++ rank-one solver on the GPU (nomad_ccd): returns W, H, obj[iters*k],
+    rmse[iters*k] (NaN where skipped; rmse all NaN without a test set), stats."""
+    h = hip()
+    W = np.array(W, np.float64, copy=True, order="C")
+    H = np.array(H, np.float64, copy=True, order="C")
+    k = W.shape[1]
+    p = CcdParams(train.m, train.n, k, iters, inner, device, lam, eps)
+    obj = np.zeros(iters * k)
+    rmse = np.zeros(iters * k)
+    st = CcdStats()
+    cp, ri, va = (np.ascontiguousarray(train.colptr, np.int64), np.ascontiguousarray(train.rowidx, np.int32),
+                  np.ascontiguousarray(train.val, np.float64))
+    if test is not None:
+        tcp, tri, tva = (np.ascontiguousarray(test.colptr, np.int64), np.ascontiguousarray(test.rowidx, np.int32),
+                         np.ascontiguousarray(test.val, np.float64))
+        targs = (_ptr(tcp, C.c_int64), _ptr(tri, C.c_int32), _ptr(tva, C.c_double), int(tcp[-1]))
+    else:
+        targs = (None, None, None, 0)
+    rc = h.nomad_ccd(C.byref(p), _ptr(cp, C.c_int64), _ptr(ri, C.c_int32), _ptr(va, C.c_double), int(cp[-1]),
+                     *targs, _ptr(W, C.c_double), _ptr(H, C.c_double), _ptr(obj, C.c_double),
+                     _ptr(rmse, C.c_double), C.byref(st))
+    if rc != 0:
+        raise NomadError(rc, h.nomad_last_error().decode())
+    return W, H, obj, rmse, st
+
+
+class Objective:
+    """The factorisation objective and its gradient on the GPU
+    (nomad_obj_create / nomad_obj_eval).  eval() takes and returns device
+    addresses (e.g. torch tensors' data_ptr()) of m x ldf / n x ldf arrays."""
+
+    def __init__(self, train: Csc, k: int, storage: str = "fp64", device: int = 0):
+        self._lib = hip()
+        self.m, self.n, self.k = train.m, train.n, k
+        self.ldf = (k + 1) & ~1
+        self.storage = storage
+        p = ObjParams(train.m, train.n, k, STORAGE_NAMES[storage], device)
+        cp, ri, va = (np.ascontiguousarray(train.colptr, np.int64), np.ascontiguousarray(train.rowidx, np.int32),
+                      np.ascontiguousarray(train.val, np.float64))
+        self._h = C.c_void_p()
+        rc = self._lib.nomad_obj_create(C.byref(p), _ptr(cp, C.c_int64), _ptr(ri, C.c_int32), _ptr(va, C.c_double),
+                                        int(cp[-1]), C.byref(self._h))
+        if rc != 0:
+            raise NomadError(rc, self._lib.nomad_last_error().decode())
+
+    def eval(self, W_ptr: int, H_ptr: int, reg: float, gW_ptr: int, gH_ptr: int, stream: int = 0) -> float:
+        loss = C.c_double()
+        rc = self._lib.nomad_obj_eval(self._h, C.c_void_p(W_ptr), C.c_void_p(H_ptr), reg, C.c_void_p(gW_ptr),
+                                      C.c_void_p(gH_ptr), C.byref(loss), C.c_void_p(stream or None))
+        if rc != 0:
+            raise NomadError(rc, self._lib.nomad_last_error().decode())
+        return loss.value
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.nomad_obj_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------
