@@ -210,6 +210,11 @@ int nomad_obj_create(const nomad_obj_params* params, const int64_t* colptr, cons
 int nomad_obj_eval(nomad_obj_ctx* ctx, const void* W, const void* H, double reg, void* gW, void* gH,
                    double* loss, void* stream);
 int nomad_obj_destroy(nomad_obj_ctx* ctx);
+/* The same evaluation for host arrays: W (m x k), H (n x k) row-major double
+ * (converted to the storage type on the device), gradients returned in the
+ * same shapes. */
+int nomad_obj_eval_host(nomad_obj_ctx* ctx, const double* W, const double* H, double reg, double* gW,
+                        double* gH, double* loss);
 
 #ifdef __cplusplus
 }

@@ -181,6 +181,7 @@ struct nomad_obj_ctx {
     double* nrm = nullptr;    // max(m, n) x ldf
     double* red = nullptr;    // kRedBlocks
     double* out = nullptr;    // 4 slots
+    void *hW = nullptr, *hH = nullptr, *hgW = nullptr, *hgH = nullptr;  // staging of nomad_obj_eval_host
     ~nomad_obj_ctx() {
         for (void* b : bufs) (void)hipFree(b);
     }
@@ -324,4 +325,52 @@ extern "C" int nomad_obj_eval(nomad_obj_ctx* c, const void* W, const void* H, do
                                   static_cast<double*>(gW), static_cast<double*>(gH), loss, s);
     return obj_eval_t<float>(c, static_cast<const float*>(W), static_cast<const float*>(H), reg,
                              static_cast<float*>(gW), static_cast<float*>(gH), loss, s);
+}
+
+// Host-array convenience: W (m x k), H (n x k) row-major double in, the
+// gradients (same shapes) out; staged through device buffers of the storage
+// type owned by the context.
+extern "C" int nomad_obj_eval_host(nomad_obj_ctx* c, const double* W, const double* H, double reg, double* gW,
+                                   double* gH, double* loss) {
+    using nomad_host::fail;
+    if (!c || !W || !H || !gW || !gH) return fail(NOMAD_EINVAL, "nomad_obj_eval_host: null argument");
+    if (hipSetDevice(c->p.device) != hipSuccess) return fail(NOMAD_EHIP, "nomad_obj_eval_host: hipSetDevice");
+    const int m = c->p.m, n = c->p.n, k = c->p.k, ldf = c->ldf;
+    int rc;
+    if (!c->hW) {
+        if ((rc = c->alloc(reinterpret_cast<char**>(&c->hW), size_t(m) * ldf * c->esz)) ||
+            (rc = c->alloc(reinterpret_cast<char**>(&c->hH), size_t(n) * ldf * c->esz)) ||
+            (rc = c->alloc(reinterpret_cast<char**>(&c->hgW), size_t(m) * ldf * c->esz)) ||
+            (rc = c->alloc(reinterpret_cast<char**>(&c->hgH), size_t(n) * ldf * c->esz)))
+            return rc;
+    }
+    auto put = [&](void* dst, const double* src, int rows) -> int {
+        std::vector<char> buf(size_t(rows) * ldf * c->esz, 0);
+        for (int r = 0; r < rows; ++r)
+            for (int d = 0; d < k; ++d) {
+                const double v = src[size_t(r) * k + d];
+                if (c->esz == 8)
+                    reinterpret_cast<double*>(buf.data())[size_t(r) * ldf + d] = v;
+                else
+                    reinterpret_cast<float*>(buf.data())[size_t(r) * ldf + d] = float(v);
+            }
+        return hipMemcpy(dst, buf.data(), buf.size(), hipMemcpyHostToDevice) == hipSuccess
+                   ? NOMAD_OK
+                   : fail(NOMAD_EHIP, "nomad_obj_eval_host: copy to device failed");
+    };
+    auto get = [&](double* dst, const void* src, int rows) -> int {
+        std::vector<char> buf(size_t(rows) * ldf * c->esz);
+        if (hipMemcpy(buf.data(), src, buf.size(), hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(NOMAD_EHIP, "nomad_obj_eval_host: copy to host failed");
+        for (int r = 0; r < rows; ++r)
+            for (int d = 0; d < k; ++d)
+                dst[size_t(r) * k + d] =
+                    c->esz == 8 ? reinterpret_cast<const double*>(buf.data())[size_t(r) * ldf + d]
+                                : double(reinterpret_cast<const float*>(buf.data())[size_t(r) * ldf + d]);
+        return NOMAD_OK;
+    };
+    if ((rc = put(c->hW, W, m)) || (rc = put(c->hH, H, n))) return rc;
+    if ((rc = nomad_obj_eval(c, c->hW, c->hH, reg, c->hgW, c->hgH, loss, nullptr))) return rc;
+    if ((rc = get(gW, c->hgW, m)) || (rc = get(gH, c->hgH, n))) return rc;
+    return NOMAD_OK;
 }

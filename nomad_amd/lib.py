@@ -103,6 +103,7 @@ HIP_SYMBOLS = {
     "nomad_obj_create": (C.c_int, [C.POINTER(ObjParams), _i64p, _i32p, _f64p, C.c_int64, C.POINTER(_P)]),
     "nomad_obj_eval": (C.c_int, [_P, _P, _P, C.c_double, _P, _P, C.POINTER(C.c_double), _P]),
     "nomad_obj_destroy": (C.c_int, [_P]),
+    "nomad_obj_eval_host": (C.c_int, [_P, _f64p, _f64p, C.c_double, _f64p, _f64p, C.POINTER(C.c_double)]),
     "nomad_ccd": (C.c_int, [C.POINTER(CcdParams), _i64p, _i32p, _f64p, C.c_int64, _i64p, _i32p, _f64p, C.c_int64,
                             _f64p, _f64p, _f64p, _f64p, C.POINTER(CcdStats)]),
 }
@@ -290,8 +291,10 @@ def ccd(train: Csc, W, H, lam: float, iters: int, inner: int, eps: float = 1e-3,
 
 class Objective:
     """The factorisation objective and its gradient on the GPU
-    (nomad_obj_create / nomad_obj_eval).  eval() takes and returns device
-    addresses (e.g. torch tensors' data_ptr()) of m x ldf / n x ldf arrays."""
+    (nomad_obj_create / nomad_obj_eval[_host]).  eval_host() takes and
+    returns host arrays; eval() takes device addresses of m x ldf / n x ldf
+    arrays allocated through this library's own HIP runtime (torch wheels
+    bundle a different one: hand torch tensors over with eval_host)."""
 
     def __init__(self, train: Csc, k: int, storage: str = "fp64", device: int = 0):
         self._lib = hip()
@@ -314,6 +317,19 @@ class Objective:
         if rc != 0:
             raise NomadError(rc, self._lib.nomad_last_error().decode())
         return loss.value
+
+    def eval_host(self, W, H, reg: float):
+        """Loss and gradients for host arrays W (m x k), H (n x k)."""
+        W = np.ascontiguousarray(W, np.float64)
+        H = np.ascontiguousarray(H, np.float64)
+        gW = np.zeros_like(W)
+        gH = np.zeros_like(H)
+        loss = C.c_double()
+        rc = self._lib.nomad_obj_eval_host(self._h, _ptr(W, C.c_double), _ptr(H, C.c_double), reg,
+                                           _ptr(gW, C.c_double), _ptr(gH, C.c_double), C.byref(loss))
+        if rc != 0:
+            raise NomadError(rc, self._lib.nomad_last_error().decode())
+        return loss.value, gW, gH
 
     def close(self) -> None:
         if self._h:

@@ -1,13 +1,15 @@
-"""torch.autograd binding of the GPU objective (nomad_obj_eval).
+"""torch.autograd binding of the GPU objective (nomad_obj_eval_host).
 
     obj = lib.Objective(train, k, storage="fp64")
-    W = torch.randn(m, obj.ldf, dtype=torch.float64, device="cuda", requires_grad=True)
-    H = ...
-    L = factor_objective(W, H, obj, reg)      # 0-d tensor on the device
-    L.backward()                               # W.grad, H.grad from the same kernels
+    W = torch.rand(m, k, dtype=torch.float64, requires_grad=True)   # any device
+    H = torch.rand(n, k, dtype=torch.float64, requires_grad=True)
+    L = factor_objective(W, H, obj, reg)
+    L.backward()                                   # W.grad, H.grad from the same kernels
 
-The forward pass computes the loss and both gradients in one call on the
-tensors' current stream; backward scales the saved gradients.
+The forward pass computes the loss and both gradients with one call of the
+hand-written kernels; backward scales the saved gradients.  torch wheels
+bundle their own HIP runtime, distinct from the one this library links, so
+tensors are handed over through host memory rather than by device address.
 """
 from __future__ import annotations
 
@@ -15,17 +17,13 @@ import torch
 
 from . import lib
 
-_DTYPES = {"fp64": torch.float64, "fp32": torch.float32}
-
 
 class _FactorObjective(torch.autograd.Function):
     @staticmethod
     def forward(ctx, W, H, obj, reg):
-        gW = torch.empty_like(W)
-        gH = torch.empty_like(H)
-        stream = torch.cuda.current_stream(W.device).cuda_stream
-        loss = obj.eval(W.data_ptr(), H.data_ptr(), float(reg), gW.data_ptr(), gH.data_ptr(), stream)
-        ctx.save_for_backward(gW, gH)
+        loss, gW, gH = obj.eval_host(W.detach().double().cpu().numpy(), H.detach().double().cpu().numpy(),
+                                     float(reg))
+        ctx.save_for_backward(torch.from_numpy(gW).to(W.device, W.dtype), torch.from_numpy(gH).to(H.device, H.dtype))
         return torch.tensor(loss, dtype=W.dtype, device=W.device)
 
     @staticmethod
@@ -35,14 +33,8 @@ class _FactorObjective(torch.autograd.Function):
 
 
 def factor_objective(W: torch.Tensor, H: torch.Tensor, obj: lib.Objective, reg: float) -> torch.Tensor:
-    """L = sum (w_i.h_j - r)^2 + reg (sum deg(i)|w_i|^2 + sum deg(j)|h_j|^2) with
-    W (m x ldf), H (n x ldf) contiguous device tensors of the objective's
-    storage type (padding columns zero)."""
-    for name, t, rows in (("W", W, obj.m), ("H", H, obj.n)):
-        if not t.is_cuda or not t.is_contiguous():
-            raise ValueError(f"{name} must be a contiguous device tensor")
-        if t.shape != (rows, obj.ldf):
-            raise ValueError(f"{name} must have shape ({rows}, {obj.ldf}), got {tuple(t.shape)}")
-        if t.dtype != _DTYPES[obj.storage]:
-            raise ValueError(f"{name} must be {_DTYPES[obj.storage]}")
+    """L = sum (w_i.h_j - r)^2 + reg (sum deg(i)|w_i|^2 + sum deg(j)|h_j|^2) for
+    W (m x k), H (n x k)."""
+    if W.shape != (obj.m, obj.k) or H.shape != (obj.n, obj.k):
+        raise ValueError(f"W must be ({obj.m}, {obj.k}) and H ({obj.n}, {obj.k})")
     return _FactorObjective.apply(W, H, obj, reg)
