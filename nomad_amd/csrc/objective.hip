@@ -108,19 +108,21 @@ __global__ void obj_pass(const ObjTask* tasks, int ntasks, const int32_t* idx, c
 }
 
 // grad[line] = 2 sum of the line's task partials + 2 reg deg(line) own[line];
-// nrm[line] = deg(line) |own[line]|^2 (FP64).  One thread per (line, element).
+// nrm[line] = deg(line) |own[line]|^2 (FP64).  Grid-stride over (line, element).
 template <class A>
 __global__ void obj_finalize(int nlines, int ldf, const int32_t* task_ptr, const int64_t* ptr, const A* part,
                              const A* own, A reg, A* grad, double* nrm) {
-    const int64_t gid = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if (gid >= int64_t(nlines) * ldf) return;
-    const int c = int(gid / ldf), d = int(gid % ldf);
-    A s = A(0);
-    for (int t = task_ptr[c]; t < task_ptr[c + 1]; ++t) s += part[int64_t(t) * ldf + d];
-    const A deg = A(ptr[c + 1] - ptr[c]);
-    const A x = own[gid];
-    grad[gid] = A(2) * s + A(2) * reg * deg * x;
-    if (nrm) nrm[gid] = double(deg) * double(x) * double(x);
+    const int64_t total = int64_t(nlines) * ldf;
+    for (int64_t gid = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; gid < total;
+         gid += int64_t(gridDim.x) * blockDim.x) {
+        const int c = int(gid / ldf), d = int(gid % ldf);
+        A s = A(0);
+        for (int t = task_ptr[c]; t < task_ptr[c + 1]; ++t) s += part[int64_t(t) * ldf + d];
+        const A deg = A(ptr[c + 1] - ptr[c]);
+        const A x = own[gid];
+        grad[gid] = A(2) * s + A(2) * reg * deg * x;
+        if (nrm) nrm[gid] = double(deg) * double(x) * double(x);
+    }
 }
 
 __global__ void obj_sum_stage1(int64_t n, const double* a, double* part) {
