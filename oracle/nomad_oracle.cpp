@@ -350,6 +350,18 @@ static void strat_replay_bf16(int rounding, uint32_t sr_seed, int m, int n, int 
     }
 }
 
+// One element pair as the kernels store it in BF16: element 2l from the low
+// 16 bits of sr_bits(seed, a, b, l), element 2l+1 from the high 16 bits
+// (stochastic), or round to nearest even.
+void orc_bf16_store_pair(float x0, float x1, int rounding, uint32_t seed, uint32_t a, uint32_t b, uint32_t lane,
+                         float* out) {
+    const uint32_t rnd = sr_bits(seed, a, b, lane);
+    const uint16_t u0 = rounding == ORC_ROUND_STOCHASTIC ? bf16_sr(x0, rnd) : bf16_rne(x0);
+    const uint16_t u1 = rounding == ORC_ROUND_STOCHASTIC ? bf16_sr(x1, rnd >> 16) : bf16_rne(x1);
+    out[0] = bf16_to_f(u0);
+    out[1] = bf16_to_f(u1);
+}
+
 int64_t orc_sgd_strat(int storage, int arith, int rounding, uint32_t sr_seed, int m, int n, int k,
                       const int64_t* colptr, const int32_t* rowidx, const double* val, double* W, double* H,
                       int B, int epoch0, int nepochs, double lrate, double drate, double reg) {

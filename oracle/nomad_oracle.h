@@ -61,6 +61,10 @@ int64_t orc_sgd_strat(int storage, int arith, int rounding, uint32_t sr_seed, in
                       const int64_t* colptr, const int32_t* rowidx, const double* val, double* W, double* H,
                       int B, int epoch0, int nepochs, double lrate, double drate, double reg);
 
+/* --- one BF16 store of an element pair, as the kernels do it -------------- */
+void orc_bf16_store_pair(float x0, float x1, int rounding, uint32_t seed, uint32_t a, uint32_t b, uint32_t lane,
+                         float* out);
+
 /* --- asynchronous NOMAD on CPU threads (the CPU baseline) ------------------ *
  * T threads, one row part each; column tokens in per-thread FIFO queues with
  * the reference's routing (nb_main.cpp:480-575).  Stops after every column

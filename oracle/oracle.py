@@ -39,6 +39,8 @@ _SYMS = {
                                 C.POINTER(AsyncStats)]),
     "orc_dat_part_info": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _i32p, _i32p, _i64p, _i32p, _i32p, _i64p]),
     "orc_dat_part_read": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _i32p, _i32p, _f64p]),
+    "orc_bf16_store_pair": (None, [C.c_float, C.c_float, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                   C.POINTER(C.c_float)]),
     "orc_sse": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _i64p, _i32p, _f64p, _f64p, _f64p]),
     "orc_loss_grad": (C.c_double, [C.c_int, C.c_int, C.c_int, _i64p, _i32p, _f64p, _f64p, _f64p, C.c_double,
                                    _f64p, _f64p]),
@@ -89,6 +91,12 @@ def dat_part(path, part, nparts):
     if rc:
         raise RuntimeError(f"orc_dat_part_read failed ({rc})")
     return m.value, n.value, tot.value, mn.value, col, rows, vals
+
+
+def bf16_store_pair(x0, x1, rounding, seed, a, b, lane):
+    out = (C.c_float * 2)()
+    lib().orc_bf16_store_pair(x0, x1, rounding, seed, a, b, lane, out)
+    return out[0], out[1]
 
 
 def init_factors(m, n, k, nthreads, seed):

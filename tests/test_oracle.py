@@ -270,3 +270,30 @@ def test_ccd_matches_reference_ccdr1_trace():
     # ranks the reference did not print were skipped by its early stop
     printed = {(p["iter"] - 1) * k + p["rank"] - 1 for p in c["trace"]}
     assert all(np.isnan(obj[q]) for q in range(len(obj)) if q not in printed)
+
+
+def test_stochastic_rounding_unbiased():
+    """The kernels' stochastic rounding (bit-identical to this restatement,
+    tests/test_gpu_sgd.py) rounds up with probability equal to the fraction
+    of the bf16 ulp below which the value lies: over 40,000 hashed draws the
+    mean of the rounded values matches the value within 4 sigma, for both
+    halves of the random word; nearest-even rounding is biased at the same
+    points."""
+    rng = np.random.default_rng(11)
+    for frac in (0.1, 0.3, 0.5, 0.77):
+        lo = np.float32(1.25)  # bf16-representable; its bf16 ulp is 2**-7
+        ulp = 2.0 ** -7
+        x = float(np.float32(lo + frac * ulp))
+        frac = (x - lo) / ulp
+        n = 20000
+        ups = np.zeros(2)
+        for t in range(n):
+            a, b, lane = (int(v) for v in rng.integers(0, 2**32, 3, dtype=np.uint64))
+            r0, r1 = O.bf16_store_pair(x, x, O.ROUND_STOCHASTIC, 0xA5A5A5A5, a, b, lane % 64)
+            assert r0 in (lo, lo + ulp) and r1 in (lo, lo + ulp)
+            ups += [(r0 > lo), (r1 > lo)]
+        p_hat = ups / n
+        sigma = np.sqrt(frac * (1 - frac) / n)
+        assert np.all(np.abs(p_hat - frac) < 4 * sigma), (frac, p_hat)
+        rne, _ = O.bf16_store_pair(x, x, O.ROUND_NEAREST, 0, 0, 0, 0)
+        assert rne == (lo + ulp if frac > 0.5 else lo)
