@@ -20,6 +20,15 @@ separate context that is then discarded.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--segment S] [--scale F]
 
 Prints one JSON line.
+
+Second workload (not the default): `--workload sweep` runs the batched
+hyper-parameter sweep -- 32 independent problems of uneven sizes (users ~
+logU(10k, 200k), items ~ logU(2k, 50k), ratings ~ logU(0.5M, 20M), k in
+{16, 32, 64, 128}, reg in {0.01, 0.05, 0.1, 0.5}), BF16 storage with
+stochastic rounding -- with each problem's strata count picked by the
+library's autotuner, then all problems trained in one persistent launch per
+step (one step = one epoch of every problem), beside the same epochs run one
+problem after another.
 """
 from __future__ import annotations
 
@@ -95,6 +104,97 @@ def cpu_baseline(tr, W0, H0, k, hyp, seconds):
                        f"{st.updates} updates")
 
 
+def sweep(a, lib) -> dict:
+    """Batched sweep: autotuned strata per problem, one launch for all."""
+    hyp = dict(lrate=0.005, drate=0.05)
+    probs = []
+    t0 = time.perf_counter()
+    for p in range(a.problems):
+        spec, k, reg = lib.preset_batch(p, a.scale)
+        tr, te = lib.synth(spec)
+        probs.append((tr, te, k, reg))
+    total = sum(x[0].nnz for x in probs)
+    log(f"sweep: generated {len(probs)} problems, {total} training ratings in {time.perf_counter() - t0:.1f} s")
+    strata, tune_s = [], time.perf_counter()
+    fixed = [int(x) for x in a.strata.split(",")] if a.strata else []
+    for i, (tr, te, k, reg) in enumerate(probs):
+        if fixed:
+            strata.append(fixed[i % len(fixed)])
+            continue
+        best, trials = lib.autotune(tr, k, mode="strat", storage="bf16", rounding="stochastic", reg=reg, seed=12345,
+                                    **hyp)
+        strata.append(best.nworkers)
+        log(f"sweep: problem {i} m={tr.m} n={tr.n} nnz={tr.nnz} k={k} reg={reg}: B={best.nworkers} "
+            f"({best.updates_per_s / 1e9:.3f} G/s; trials " +
+            ", ".join(f"{t.nworkers}:{t.updates_per_s / 1e9:.3f}" for t in trials) + ")")
+    tune_s = time.perf_counter() - tune_s
+
+    def contexts():
+        cs = []
+        for i, (tr, te, k, reg) in enumerate(probs):
+            c = lib.Context(tr.m, tr.n, k, storage="bf16", rounding="stochastic", mode="strat", nworkers=strata[i],
+                            seed=12345 + i, reg=reg, **hyp)
+            c.upload(tr, te)
+            W0, H0 = lib.init_factors(tr.m, tr.n, k, 4, 12345 + i)
+            c.set_factors(W0, H0)
+            cs.append(c)
+        return cs
+
+    batch = contexts()
+    for _ in range(a.warmup):
+        lib.train_batch(batch, 1)
+    dev = alg = 0.0
+    upd = 0
+    for step in range(a.steps):
+        st = lib.train_batch(batch, 1)
+        dev += st[0].train_seconds
+        upd += sum(x.updates for x in st)
+        alg += sum(x.alg_bytes for x in st)
+        log(f"sweep: batched epoch {step + 1}: {st[0].train_seconds * 1e3:.2f} ms")
+    seq = contexts()
+    for c in seq:
+        for _ in range(a.warmup):
+            c.train(epochs=1)
+    seq_dev = 0.0
+    for step in range(a.steps):
+        seq_dev += sum(c.train(epochs=1).train_seconds for c in seq)
+    same = all(np.array_equal(x, y) for cb, cs in zip(batch, seq) for x, y in zip(cb.get_factors(), cs.get_factors()))
+    rmse = [float(np.sqrt(c.eval().test_sse / max(pr[1].nnz, 1))) for c, pr in zip(batch, probs)]
+    for c in batch + seq:
+        c.close()
+    achieved = alg / dev / 1e9
+    return {
+        "metric": "batched sweep updates/s, 32 uneven problems, BF16 storage, 1 GPU",
+        "value": upd / dev,
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": dev / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched strat sweep, {len(probs)} problems, one epoch of each per launch",
+            "problems": [dict(m=p[0].m, n=p[0].n, nnz=p[0].nnz, k=p[2], reg=p[3], B=b) for p, b in zip(probs, strata)],
+            "storage": "bf16", "rounding": "stochastic", **hyp, "scale": a.scale,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "sgd_batch_kernel<bf16>",
+            "alg_bytes_per_launch": alg / a.steps, "avg_launch_ms": dev / a.steps * 1e3,
+        },
+        "sequential": {"updates_per_s": upd / seq_dev, "ms_per_step": seq_dev / a.steps * 1e3,
+                       "bitwise_equal_to_batched": bool(same)},
+        "speedup_vs_sequential": seq_dev / dev,
+        "autotune_seconds": tune_s,
+        "strata_source": "fixed" if fixed else "autotuner",
+        "test_rmse_after": rmse,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +206,10 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp32", "bf16"], default=None)
+    ap.add_argument("--workload", choices=["netflix", "sweep"], default="netflix")
+    ap.add_argument("--problems", type=int, default=32, help="sweep: number of problems")
+    ap.add_argument("--strata", default="", help="sweep: comma-separated strata per problem (cycled) instead of "
+                                                   "the autotuner's choice")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); multi-GPU token exchange is a design, "
@@ -115,6 +219,9 @@ def main() -> int:
     from nomad_amd import lib
 
     lib.hip()
+    if a.workload == "sweep":
+        print(json.dumps(sweep(a, lib)), flush=True)
+        return 0
     k = 100
     hyp = dict(lrate=0.012, drate=0.05, reg=0.05)
     t0 = time.perf_counter()

@@ -128,6 +128,49 @@ int nomad_get_factors(nomad_ctx* ctx, double* W, double* H);
  * of the two must be positive.  Strat mode runs `epochs` full B x B sweeps. */
 int nomad_train(nomad_ctx* ctx, double seconds, int64_t epochs, nomad_train_stats* stats);
 
+/* Batched sweep (no reference counterpart: the reference runs one problem
+ * per process; this replaces a loop over separate nomad_double runs of a
+ * hyper-parameter sweep, e.g. over --reg / --dim).  Runs `epochs` epochs of
+ * every context's own stratified schedule in ONE persistent launch.  The
+ * contexts must be in NOMAD_MODE_STRAT, on one device, with one storage
+ * type and uploaded training data; each keeps its own ratings, factors,
+ * rank, strata count and hyper-parameters, and its result is bit-identical
+ * to nomad_train(ctx, 0, epochs) on that context alone.  count <= 256.
+ * waves = persistent grid in wavefronts (0: one per tile of the widest
+ * round, capped at what fits on the device; any positive value is valid).
+ * stats (may be NULL) receives one record per context; train_seconds is
+ * the device time of the whole launch. */
+int nomad_train_batch(nomad_ctx* const* ctxs, int32_t count, int64_t epochs, int32_t waves,
+                      nomad_train_stats* stats);
+
+/* Autotuner (no reference counterpart; the reference's worker count is the
+ * --nthreads option).  Times short training trials of candidate launch
+ * configurations on the given problem and returns the fastest:
+ *   async mode: workers (row blocks) per compute unit in {2, 4, 6, 8, 12,
+ *               16} (the workgroup shape and the ring capacity follow from
+ *               the worker count), with and without NOMAD_FLAG_BALANCE;
+ *   strat mode: strata B in {1/4, 1/2, 1, 2, 4} x compute units (at most
+ *               m), timed through nomad_train_batch with the default grid.
+ * params gives the problem and hyper-parameters; its nworkers and the
+ * BALANCE flag are ignored.  Each trial trains a scratch context from
+ * factors drawn from mt19937_64(params->seed), U(0, 1/sqrt(k)): async
+ * trials run two segments of trial_seconds (the second is timed), strat
+ * trials one warm-up and one timed epoch.  trials (may be NULL) receives up
+ * to max_trials records in the order run; *ntrials the number run.  A
+ * trial whose context cannot be created records its error code and is
+ * skipped; the call fails only if every trial fails. */
+typedef struct nomad_tune_trial {
+    int32_t nworkers;      /* workers (async) or strata B (strat) actually used */
+    int32_t flags;         /* NOMAD_FLAG_BALANCE or 0                           */
+    int32_t status;        /* NOMAD_OK or the trial's error code                */
+    int32_t pad;
+    double updates_per_s;  /* timed rate (0 for a failed trial)                 */
+} nomad_tune_trial;
+
+int nomad_autotune(const nomad_params* params, const int64_t* colptr, const int32_t* rowidx, const double* val,
+                   int64_t nnz, double trial_seconds, nomad_tune_trial* best, nomad_tune_trial* trials,
+                   int32_t max_trials, int32_t* ntrials);
+
 /* Sum of squared errors over the training and test ratings, accumulated in
  * FP64 (the dot product runs in the storage's arithmetic type). */
 int nomad_eval(nomad_ctx* ctx, nomad_eval_stats* stats);

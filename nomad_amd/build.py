@@ -40,7 +40,8 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", f"-I{CSRC}"]
 
-HIP_SOURCES = ["sgd_kernels.hip", "async_kernel.hip", "eval_kernels.hip", "ccd.hip", "objective.hip", "capi.hip"]
+HIP_SOURCES = ["sgd_kernels.hip", "async_kernel.hip", "eval_kernels.hip", "ccd.hip", "objective.hip", "capi.hip",
+               "autotune.hip"]
 TOOLS_SOURCES = ["datfile.cpp", "synth.cpp", "tools_capi.cpp"]
 
 

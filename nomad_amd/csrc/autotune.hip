@@ -1,0 +1,119 @@
+// Autotuner of the launch configuration (nomad_autotune in nomad_hip.h).
+//
+// The reference fixes its worker count by the --nthreads option
+// (Code/nomad/nomad_option.cpp:16-77) and has no tuner.  Here the worker
+// count (async mode: row blocks, hence workgroup shape and ring capacity) or
+// the strata count (strat mode) is picked by timing short trials of each
+// candidate on the caller's own problem.  Only the public C ABI is used, so a
+// trial is exactly what a caller would run with that configuration.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "nomad_hip.h"
+
+namespace nomad_host {
+int fail(int code, const std::string& msg);
+}
+
+namespace {
+
+struct Candidate {
+    int32_t nworkers;
+    int32_t flags;
+};
+
+// Times one candidate: returns NOMAD_OK and the rate, or the error code.
+int run_trial(const nomad_params& base, const Candidate& cand, const int64_t* colptr, const int32_t* rowidx,
+              const double* val, int64_t nnz, const std::vector<double>& W0, const std::vector<double>& H0,
+              double trial_seconds, nomad_tune_trial* out) {
+    nomad_params p = base;
+    p.nworkers = cand.nworkers;
+    p.flags = (base.flags & ~NOMAD_FLAG_BALANCE) | cand.flags;
+    out->nworkers = cand.nworkers;
+    out->flags = cand.flags;
+    out->updates_per_s = 0.0;
+    nomad_ctx* c = nullptr;
+    int rc = nomad_create(&p, &c);
+    if (rc == NOMAD_OK) rc = nomad_upload_train(c, colptr, rowidx, val, nnz);
+    if (rc == NOMAD_OK) rc = nomad_set_factors(c, W0.data(), H0.data());
+    if (rc == NOMAD_OK) rc = nomad_get_nworkers(c, &out->nworkers);
+    nomad_train_stats st{};
+    if (rc == NOMAD_OK) {
+        if (p.mode == NOMAD_MODE_ASYNC) {
+            rc = nomad_train(c, trial_seconds, 0, &st);  // warm-up: builds the routes
+            if (rc == NOMAD_OK) rc = nomad_train(c, trial_seconds, 0, &st);
+        } else {
+            rc = nomad_train_batch(&c, 1, 1, 0, &st);
+            if (rc == NOMAD_OK) rc = nomad_train_batch(&c, 1, 1, 0, &st);
+        }
+    }
+    if (rc == NOMAD_OK && st.train_seconds > 0) out->updates_per_s = double(st.updates) / st.train_seconds;
+    out->status = rc;
+    nomad_destroy(c);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" int nomad_autotune(const nomad_params* params, const int64_t* colptr, const int32_t* rowidx,
+                              const double* val, int64_t nnz, double trial_seconds, nomad_tune_trial* best,
+                              nomad_tune_trial* trials, int32_t max_trials, int32_t* ntrials) {
+    using nomad_host::fail;
+    if (!params || !colptr || !best || (max_trials > 0 && !trials) || max_trials < 0)
+        return fail(NOMAD_EINVAL, "nomad_autotune: null or negative argument");
+    if (params->mode == NOMAD_MODE_ASYNC && !(trial_seconds > 0))
+        return fail(NOMAD_EINVAL, "nomad_autotune: async trials need trial_seconds > 0");
+    if (params->m <= 0 || params->n <= 0 || params->k <= 0)
+        return fail(NOMAD_EINVAL, "nomad_autotune: m, n, k must be positive");
+    if (ntrials) *ntrials = 0;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, params->device) != hipSuccess || cus <= 0)
+        return fail(NOMAD_EHIP, "nomad_autotune: device query failed");
+
+    std::vector<Candidate> cands;
+    if (params->mode == NOMAD_MODE_ASYNC) {
+        for (int per_cu : {2, 4, 6, 8, 12, 16})
+            for (int32_t fl : {NOMAD_FLAG_BALANCE, 0}) cands.push_back({int32_t(per_cu * cus), fl});
+    } else {
+        int32_t last = -1;
+        for (int num : {1, 2, 4, 8, 16}) {  // B = num / 4 x compute units
+            const int32_t B = std::max(1, std::min(params->m, num * cus / 4));
+            if (B != last) cands.push_back({B, 0});
+            last = B;
+        }
+    }
+
+    // one start point for every trial: U(0, 1/sqrt(k)) from the seed
+    std::mt19937_64 rng(params->seed);
+    std::uniform_real_distribution<double> u(0.0, 1.0 / std::sqrt(double(params->k)));
+    std::vector<double> W0(size_t(params->m) * params->k), H0(size_t(params->n) * params->k);
+    for (double& x : W0) x = u(rng);
+    for (double& x : H0) x = u(rng);
+
+    int32_t run = 0;
+    bool any = false;
+    int first_err = NOMAD_OK;
+    std::string first_msg;
+    for (const Candidate& cd : cands) {
+        nomad_tune_trial t{};
+        const int rc = run_trial(*params, cd, colptr, rowidx, val, nnz, W0, H0, trial_seconds, &t);
+        if (run < max_trials) trials[run] = t;
+        ++run;
+        if (rc == NOMAD_OK) {
+            if (!any || t.updates_per_s > best->updates_per_s) *best = t;
+            any = true;
+        } else if (first_err == NOMAD_OK) {
+            first_err = rc;
+            first_msg = nomad_last_error();
+        }
+    }
+    if (ntrials) *ntrials = run;
+    if (!any) return fail(first_err, "nomad_autotune: every trial failed; first error: " + first_msg);
+    return NOMAD_OK;
+}

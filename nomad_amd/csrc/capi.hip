@@ -29,6 +29,8 @@ using nomad_k::VisitDesc;
 
 template <int S>
 __global__ void sgd_strat_kernel(SgdArgs);
+template <int S>
+__global__ void sgd_batch_kernel(nomad_k::BatchArgs);
 template <int S, int NW, int SLOTS>
 __global__ void sgd_async_kernel(SgdArgs);
 template <int S>
@@ -208,6 +210,11 @@ struct nomad_ctx {
     int segments = 0;
     int64_t cycle_base = 0;      // route cycles completed by every active column
     int64_t strat_epochs = 0;
+    // strat: per tile (row block b, column block cb) its non-empty column
+    // visits, columns ascending; d_tile_ptr has B * B + 1 entries
+    bool tiles_ready = false;
+    int64_t* d_tile_ptr = nullptr;
+    VisitDesc* d_tile_visits = nullptr;
 
     ~nomad_ctx() {
         dev_free(dW);
@@ -224,6 +231,8 @@ struct nomad_ctx {
         dev_free(d_partial);
         dev_free(d_vptr);
         dev_free(d_visits);
+        dev_free(d_tile_ptr);
+        dev_free(d_tile_visits);
         dev_free(d_wbound);
         coh_free(d_tok);
         coh_free(d_tok_stamp);
@@ -305,7 +314,55 @@ int upload_csc(nomad_ctx* c, DevCsc& d, const int64_t* colptr, const int32_t* ro
         c->h_colptr.assign(colptr, colptr + n + 1);
         c->h_rowidx.assign(rowidx, rowidx + nnz);
         c->routes_ready = false;
+        c->tiles_ready = false;
     }
+    return NOMAD_OK;
+}
+
+// Tile lists of the stratified schedule: row block b = rows [b * rpb,
+// (b + 1) * rpb), column block cb = columns [cb * cpb, (cb + 1) * cpb).  The
+// visits of row block b are listed by ascending column, so tile (b, cb) is
+// the contiguous run of them whose column lies in column block cb.  One pass
+// over the ratings; a final sentinel visit (len 0) lets a tile loop read one
+// descriptor ahead without a bounds test.
+int ensure_tiles(nomad_ctx* c) {
+    if (c->tiles_ready) return NOMAD_OK;
+    const int32_t n = c->p.n, B = c->P;
+    std::vector<std::vector<VisitDesc>> per(static_cast<size_t>(B));
+    for (int32_t j = 0; j < n; ++j) {
+        const int64_t c0 = c->h_colptr[j], c1 = c->h_colptr[j + 1];
+        int64_t e = c0;
+        while (e < c1) {
+            const int32_t b = c->h_rowidx[e] / c->rpb;
+            int64_t f = e + 1;
+            while (f < c1 && c->h_rowidx[f] / c->rpb == b) ++f;
+            per[b].push_back(VisitDesc{e, int32_t(f - e), j});
+            e = f;
+        }
+    }
+    std::vector<int64_t> tptr(size_t(B) * B + 1, 0);
+    std::vector<VisitDesc> vis;
+    size_t total = 0;
+    for (const auto& v : per) total += v.size();
+    vis.reserve(total + 1);
+    for (int32_t b = 0; b < B; ++b) {
+        size_t x = 0;
+        const auto& v = per[b];
+        for (int32_t cb = 0; cb < B; ++cb) {
+            tptr[size_t(b) * B + cb] = int64_t(vis.size());
+            const int32_t chi = std::min<int64_t>(n, int64_t(cb + 1) * c->cpb);
+            while (x < v.size() && v[x].worker < chi) vis.push_back(v[x++]);
+        }
+    }
+    tptr[size_t(B) * B] = int64_t(vis.size());
+    vis.push_back(VisitDesc{0, 0, 0});
+    dev_free(c->d_tile_ptr);
+    dev_free(c->d_tile_visits);
+    int rc;
+    if ((rc = dev_alloc(&c->d_tile_ptr, tptr.size())) || (rc = dev_alloc(&c->d_tile_visits, vis.size()))) return rc;
+    NOMAD_HIP_TRY(hipMemcpy(c->d_tile_ptr, tptr.data(), tptr.size() * 8, hipMemcpyHostToDevice));
+    NOMAD_HIP_TRY(hipMemcpy(c->d_tile_visits, vis.data(), vis.size() * sizeof(VisitDesc), hipMemcpyHostToDevice));
+    c->tiles_ready = true;
     return NOMAD_OK;
 }
 
@@ -480,6 +537,10 @@ SgdArgs base_args(nomad_ctx* c) {
     a.rowidx = c->tr.rowidx;
     a.val = c->tr.val;
     a.steps = c->d_steps;
+    if (c->p.mode == NOMAD_MODE_STRAT) {
+        a.tile_ptr = c->d_tile_ptr;
+        a.visits = c->d_tile_visits;
+    }
     a.lrate = c->p.storage == NOMAD_STORE_FP64 ? c->p.lrate : double(float(c->p.lrate));
     a.drate = c->p.storage == NOMAD_STORE_FP64 ? c->p.drate : double(float(c->p.drate));
     a.reg = c->p.storage == NOMAD_STORE_FP64 ? c->p.reg : double(float(c->p.reg));
@@ -490,6 +551,17 @@ template <int S>
 hipError_t launch_strat(const SgdArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(sgd_strat_kernel<S>, dim3(a.nblocks), dim3(64), 0, st, a);
     return hipGetLastError();
+}
+
+template <int S>
+hipError_t launch_batch(const nomad_k::BatchArgs& a, int blocks, hipStream_t st) {
+    hipLaunchKernelGGL(sgd_batch_kernel<S>, dim3(blocks), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int S>
+hipError_t batch_occupancy(int* per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, sgd_batch_kernel<S>, 256, 0);
 }
 
 template <int S>
@@ -655,10 +727,36 @@ int train_async(nomad_ctx* c, double seconds, int64_t epochs, nomad_train_stats*
     return NOMAD_OK;
 }
 
+// Statistics of `epochs` stratified epochs: every rating once per epoch;
+// visits = non-empty (column, row block) pairs.
+void strat_stats(const nomad_ctx* c, int64_t epochs, double seconds, nomad_train_stats* st) {
+    if (!st) return;
+    std::memset(st, 0, sizeof(*st));
+    int64_t vis = 0;
+    for (int32_t j = 0; j < c->p.n; ++j) {
+        int32_t last = -1;
+        for (int64_t e = c->h_colptr[j]; e < c->h_colptr[j + 1]; ++e) {
+            const int32_t b = c->h_rowidx[e] / c->rpb;
+            if (b != last) {
+                ++vis;
+                last = b;
+            }
+        }
+    }
+    st->updates = c->tr.nnz * epochs;
+    st->visits = vis * epochs;
+    st->failed_pops = 0;
+    st->epochs_done = c->strat_epochs;
+    st->train_seconds = seconds;
+    const double ks = double(c->p.k) * double(c->esz);
+    st->alg_bytes = double(st->updates) * (4.0 + double(c->rvsz) + 2.0 * ks) + double(st->visits) * 2.0 * ks;
+}
+
 int train_strat(nomad_ctx* c, int64_t epochs, nomad_train_stats* st) {
     if (epochs <= 0) return fail(NOMAD_EINVAL, "strat mode needs epochs > 0");
     int rc = ensure_steps(c, c->strat_epochs + epochs + 2);
     if (rc) return rc;
+    if ((rc = ensure_tiles(c))) return rc;
     SgdArgs a = base_args(c);
     NOMAD_HIP_TRY(hipEventRecord(c->ev0, c->stream));
     for (int64_t ep = 0; ep < epochs; ++ep) {
@@ -675,26 +773,152 @@ int train_strat(nomad_ctx* c, int64_t epochs, nomad_train_stats* st) {
     float ms = 0.f;
     NOMAD_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->strat_epochs += epochs;
-    if (st) {
-        // every rating once per epoch; visits = non-empty (column, row block) pairs
-        int64_t vis = 0;
-        for (int32_t j = 0; j < c->p.n; ++j) {
-            int32_t last = -1;
-            for (int64_t e = c->h_colptr[j]; e < c->h_colptr[j + 1]; ++e) {
-                const int32_t b = c->h_rowidx[e] / c->rpb;
-                if (b != last) {
-                    ++vis;
-                    last = b;
-                }
-            }
+    strat_stats(c, epochs, ms * 1e-3, st);
+    return NOMAD_OK;
+}
+
+// Batched stratified training (see sgd_batch_kernel): every context runs
+// `epochs` epochs of its own B x B schedule in one persistent launch.
+int train_batch(nomad_ctx* const* cs, int32_t count, int64_t epochs, int32_t waves, nomad_train_stats* stats) {
+    using nomad_k::BatchArgs;
+    if (!cs || count <= 0 || count > nomad_k::kBatchMax || epochs <= 0 || waves < 0)
+        return fail(NOMAD_EINVAL, "nomad_train_batch: need 1..256 contexts, epochs > 0 and waves >= 0");
+    for (int32_t i = 0; i < count; ++i) {
+        const nomad_ctx* c = cs[i];
+        if (!c) return fail(NOMAD_EINVAL, "nomad_train_batch: null context");
+        if (c->p.mode != NOMAD_MODE_STRAT) return fail(NOMAD_EINVAL, "nomad_train_batch: contexts must be in strat mode");
+        if (c->p.device != cs[0]->p.device || c->p.storage != cs[0]->p.storage)
+            return fail(NOMAD_EINVAL, "nomad_train_batch: contexts must share device and storage type");
+        if (c->tr.nnz < 0) return fail(NOMAD_ESTATE, "nomad_train_batch: a context has no training data");
+        for (int32_t j = 0; j < i; ++j)
+            if (cs[j] == c) return fail(NOMAD_EINVAL, "nomad_train_batch: a context appears twice");
+        if (epochs * int64_t(c->P) >= (int64_t(1) << 31))
+            return fail(NOMAD_EINVAL, "nomad_train_batch: epochs * strata exceeds 2^31 rounds");
+    }
+    nomad_ctx* c0 = cs[0];
+    NOMAD_HIP_TRY(hipSetDevice(c0->p.device));
+    std::vector<SgdArgs> probs(static_cast<size_t>(count));
+    std::vector<int64_t> rounds(static_cast<size_t>(count)), done_off(static_cast<size_t>(count));
+    int64_t ndone = 0;
+    for (int32_t i = 0; i < count; ++i) {
+        nomad_ctx* c = cs[i];
+        int rc = ensure_steps(c, c->strat_epochs + epochs + 2);
+        if (rc) return rc;
+        if ((rc = ensure_tiles(c))) return rc;
+        NOMAD_HIP_TRY(hipStreamSynchronize(c->stream));
+        probs[i] = base_args(c);
+        probs[i].epoch = int32_t(c->strat_epochs);
+        probs[i].substep = 0;
+        rounds[i] = epochs * c->P;
+        done_off[i] = ndone;
+        ndone += c->P;
+    }
+    // problems sorted by rounds, descending: those still running in a round
+    // are a prefix of this order
+    std::vector<int32_t> order(static_cast<size_t>(count));
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return rounds[x] > rounds[y]; });
+    std::vector<int32_t> boff(static_cast<size_t>(count) + 1, 0);
+    for (int32_t q = 0; q < count; ++q) boff[q + 1] = boff[q] + cs[order[q]]->P;
+    std::vector<int64_t> seg_t0;
+    std::vector<int32_t> seg_r0, seg_w;
+    int64_t t = 0, r_prev = 0;
+    for (int32_t A = count; A >= 1; --A) {  // A problems running in rounds [r_prev, rounds of sorted A-1)
+        const int64_t r_end = rounds[order[A - 1]];
+        if (r_end <= r_prev) continue;
+        seg_t0.push_back(t);
+        seg_r0.push_back(int32_t(r_prev));
+        seg_w.push_back(boff[A]);
+        t += (r_end - r_prev) * boff[A];
+        r_prev = r_end;
+    }
+    const int64_t nitems = t;
+
+    int cus = 0, per_cu = 0;
+    NOMAD_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c0->p.device));
+    NOMAD_HIP_TRY(dispatch_storage(c0->p.storage, [&](auto tag) {
+        return batch_occupancy<decltype(tag)::value>(&per_cu);
+    }));
+    // default grid: one wave per tile of the widest round, up to the waves
+    // that fit on the device at once (more would only wait)
+    int64_t want = waves > 0 ? waves : std::min<int64_t>(boff[count], int64_t(cus) * std::max(per_cu, 1) * 4);
+    const int blocks = int(std::max<int64_t>(1, std::min<int64_t>((want + 3) / 4, 65535)));
+
+    struct Scratch {
+        SgdArgs* probs = nullptr;
+        int64_t* seg_t0 = nullptr;
+        int32_t* seg_r0 = nullptr;
+        int32_t* seg_w = nullptr;
+        int32_t* boff = nullptr;
+        int32_t* order = nullptr;
+        int64_t* done_off = nullptr;
+        uint32_t* done = nullptr;
+        uint64_t* ticket = nullptr;
+        uint32_t* timed_out = nullptr;
+        ~Scratch() {
+            dev_free(probs);
+            dev_free(seg_t0);
+            dev_free(seg_r0);
+            dev_free(seg_w);
+            dev_free(boff);
+            dev_free(order);
+            dev_free(done_off);
+            coh_free(done);
+            coh_free(ticket);
+            coh_free(timed_out);
         }
-        st->updates = c->tr.nnz * epochs;
-        st->visits = vis * epochs;
-        st->failed_pops = 0;
-        st->epochs_done = c->strat_epochs;
-        st->train_seconds = ms * 1e-3;
-        const double ks = double(c->p.k) * double(c->esz);
-        st->alg_bytes = double(st->updates) * (4.0 + double(c->rvsz) + 2.0 * ks) + double(st->visits) * 2.0 * ks;
+    } w;
+    int rc;
+    const size_t nseg = seg_t0.size();
+    if ((rc = dev_alloc(&w.probs, size_t(count))) || (rc = dev_alloc(&w.seg_t0, nseg)) ||
+        (rc = dev_alloc(&w.seg_r0, nseg)) || (rc = dev_alloc(&w.seg_w, nseg)) ||
+        (rc = dev_alloc(&w.boff, boff.size())) || (rc = dev_alloc(&w.order, order.size())) ||
+        (rc = dev_alloc(&w.done_off, done_off.size())) || (rc = coh_alloc(c0->p.device, &w.done, size_t(ndone))) ||
+        (rc = coh_alloc(c0->p.device, &w.ticket, 1)) || (rc = coh_alloc(c0->p.device, &w.timed_out, 1)))
+        return rc;
+    NOMAD_HIP_TRY(hipMemcpy(w.probs, probs.data(), probs.size() * sizeof(SgdArgs), hipMemcpyHostToDevice));
+    NOMAD_HIP_TRY(hipMemcpy(w.seg_t0, seg_t0.data(), nseg * 8, hipMemcpyHostToDevice));
+    NOMAD_HIP_TRY(hipMemcpy(w.seg_r0, seg_r0.data(), nseg * 4, hipMemcpyHostToDevice));
+    NOMAD_HIP_TRY(hipMemcpy(w.seg_w, seg_w.data(), nseg * 4, hipMemcpyHostToDevice));
+    NOMAD_HIP_TRY(hipMemcpy(w.boff, boff.data(), boff.size() * 4, hipMemcpyHostToDevice));
+    NOMAD_HIP_TRY(hipMemcpy(w.order, order.data(), order.size() * 4, hipMemcpyHostToDevice));
+    NOMAD_HIP_TRY(hipMemcpy(w.done_off, done_off.data(), done_off.size() * 8, hipMemcpyHostToDevice));
+    NOMAD_HIP_TRY(hipMemsetAsync(w.done, 0, size_t(ndone) * 4, c0->stream));
+    NOMAD_HIP_TRY(hipMemsetAsync(w.ticket, 0, 8, c0->stream));
+    NOMAD_HIP_TRY(hipMemsetAsync(w.timed_out, 0, 4, c0->stream));
+
+    BatchArgs ba{};
+    ba.probs = w.probs;
+    ba.seg_t0 = w.seg_t0;
+    ba.seg_r0 = w.seg_r0;
+    ba.seg_w = w.seg_w;
+    ba.boff = w.boff;
+    ba.order = w.order;
+    ba.done_off = w.done_off;
+    ba.done = w.done;
+    ba.ticket = w.ticket;
+    ba.timed_out = w.timed_out;
+    ba.nitems = nitems;
+    ba.nsorted = count;
+    ba.nseg = int32_t(nseg);
+    ba.deadline = uint64_t(watchdog_seconds() * c0->wall_hz);
+    NOMAD_HIP_TRY(hipEventRecord(c0->ev0, c0->stream));
+    hipError_t e = dispatch_storage(c0->p.storage, [&](auto tag) {
+        return launch_batch<decltype(tag)::value>(ba, blocks, c0->stream);
+    });
+    if (e != hipSuccess) return fail(NOMAD_EHIP, std::string("batch launch: ") + hipGetErrorString(e));
+    NOMAD_HIP_TRY(hipEventRecord(c0->ev1, c0->stream));
+    NOMAD_HIP_TRY(hipStreamSynchronize(c0->stream));
+    float ms = 0.f;
+    NOMAD_HIP_TRY(hipEventElapsedTime(&ms, c0->ev0, c0->ev1));
+    uint32_t timed_out = 0;
+    NOMAD_HIP_TRY(hipMemcpy(&timed_out, w.timed_out, 4, hipMemcpyDeviceToHost));
+    if (timed_out)
+        return fail(NOMAD_ETIMEOUT, "batched training stopped by the device watchdog after " +
+                                        std::to_string(watchdog_seconds()) + " s; factors are partial");
+    for (int32_t i = 0; i < count; ++i) {
+        cs[i]->strat_epochs += epochs;
+        strat_stats(cs[i], epochs, ms * 1e-3, stats ? stats + i : nullptr);
     }
     return NOMAD_OK;
 }
@@ -932,6 +1156,11 @@ int nomad_train(nomad_ctx* c, double seconds, int64_t epochs, nomad_train_stats*
     if (st) std::memset(st, 0, sizeof(*st));
     if (c->p.mode == NOMAD_MODE_STRAT) return train_strat(c, epochs, st);
     return train_async(c, seconds, epochs, st);
+}
+
+int nomad_train_batch(nomad_ctx* const* ctxs, int32_t count, int64_t epochs, int32_t waves,
+                      nomad_train_stats* stats) {
+    return train_batch(ctxs, count, epochs, waves, stats);
 }
 
 int nomad_eval(nomad_ctx* c, nomad_eval_stats* st) {

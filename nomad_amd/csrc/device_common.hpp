@@ -118,6 +118,11 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void* row, int 
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(row), 0, bytes, 0x00020000);
 }
 
+// Cache-policy operand of a buffer access that makes it device-coherent:
+// the sc1 bit, the encoding the compiler gives relaxed agent-scope atomic
+// loads and stores on gfx950 (see ld_dc / st_dc below).
+constexpr int kCpDevice = 16;
+
 // Device-coherent accesses: relaxed agent-scope atomics, i.e. loads and
 // stores with the sc1 bit, which are coherent across the XCDs' separate L2s
 // without the L2 writeback / invalidate that an agent-scope fence costs.
@@ -151,14 +156,16 @@ struct Store<NOMAD_STORE_FP32> {
     struct pair {
         float x, y;
     };
+    template <int CP = 0>
     static __device__ __forceinline__ pair load(const elem* row, int lane, int npairs) {
-        const u32x2_t v = __builtin_amdgcn_raw_buffer_load_b64(row_rsrc(row, npairs * 8), lane * 8, 0, 0);
+        const u32x2_t v = __builtin_amdgcn_raw_buffer_load_b64(row_rsrc(row, npairs * 8), lane * 8, 0, CP);
         return pair{__uint_as_float(v.x), __uint_as_float(v.y)};
     }
+    template <int CP = 0>
     static __device__ __forceinline__ void store(elem* row, int lane, int npairs, pair p, uint32_t,
                                                  int) {
         const u32x2_t v = {__float_as_uint(p.x), __float_as_uint(p.y)};
-        __builtin_amdgcn_raw_buffer_store_b64(v, row_rsrc(row, npairs * 8), lane * 8, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(v, row_rsrc(row, npairs * 8), lane * 8, 0, CP);
     }
     // rows of a block through one buffer resource: byte offset voff (per
     // lane, out of range for lanes past the row) + soff (row within block)
@@ -197,16 +204,18 @@ struct Store<NOMAD_STORE_FP64> {
     struct pair {
         double x, y;
     };
+    template <int CP = 0>
     static __device__ __forceinline__ pair load(const elem* row, int lane, int npairs) {
-        const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(row_rsrc(row, npairs * 16), lane * 16, 0, 0);
+        const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(row_rsrc(row, npairs * 16), lane * 16, 0, CP);
         return pair{__builtin_bit_cast(double, (uint64_t(v.y) << 32) | v.x),
                     __builtin_bit_cast(double, (uint64_t(v.w) << 32) | v.z)};
     }
+    template <int CP = 0>
     static __device__ __forceinline__ void store(elem* row, int lane, int npairs, pair p, uint32_t,
                                                  int) {
         const uint64_t a = __builtin_bit_cast(uint64_t, p.x), b = __builtin_bit_cast(uint64_t, p.y);
         const u32x4_t v = {uint32_t(a), uint32_t(a >> 32), uint32_t(b), uint32_t(b >> 32)};
-        __builtin_amdgcn_raw_buffer_store_b128(v, row_rsrc(row, npairs * 16), lane * 16, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b128(v, row_rsrc(row, npairs * 16), lane * 16, 0, CP);
     }
     static __device__ __forceinline__ pair rounded(pair p, uint32_t, int) { return p; }
     static constexpr int kPairBytes = 16;
@@ -246,10 +255,12 @@ struct Store<NOMAD_STORE_BF16> {
         float x, y;
     };
     // element pair (2l, 2l+1) is one aligned 32-bit word
+    template <int CP = 0>
     static __device__ __forceinline__ pair load(const elem* row, int lane, int npairs) {
-        const uint32_t u = __builtin_amdgcn_raw_buffer_load_b32(row_rsrc(row, npairs * 4), lane * 4, 0, 0);
+        const uint32_t u = __builtin_amdgcn_raw_buffer_load_b32(row_rsrc(row, npairs * 4), lane * 4, 0, CP);
         return pair{bf16_lo(u), bf16_hi(u)};
     }
+    template <int CP = 0>
     static __device__ __forceinline__ void store(elem* row, int lane, int npairs, pair p, uint32_t rnd,
                                                  int round_mode) {
         uint32_t lo, hi;
@@ -260,7 +271,7 @@ struct Store<NOMAD_STORE_BF16> {
             lo = f32_to_bf16_rne(p.x);
             hi = f32_to_bf16_rne(p.y);
         }
-        __builtin_amdgcn_raw_buffer_store_b32(lo | (hi << 16), row_rsrc(row, npairs * 4), lane * 4, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(lo | (hi << 16), row_rsrc(row, npairs * 4), lane * 4, 0, CP);
     }
     static constexpr int kPairBytes = 4;
     static __device__ __forceinline__ pair load_b(__amdgpu_buffer_rsrc_t r, int voff, int soff = 0) {

@@ -6,7 +6,8 @@
 namespace nomad_k {
 
 // One non-empty (column, worker) visit: the worker's ratings of the column are
-// the contiguous CSC range [start, start + len) (rows ascending).
+// the contiguous CSC range [start, start + len) (rows ascending).  In the
+// stratified schedule's tile lists `worker` holds the column instead.
 struct VisitDesc {
     int64_t start;
     int32_t len;
@@ -70,8 +71,28 @@ struct SgdArgs {
     uint64_t deadline;         // wall_clock64 ticks after its start at which a worker stops
 
     // strat schedule
+    const int64_t* tile_ptr;   // B * B + 1: first visit of tile (b, cb) at [b * B + cb] in visits
     int32_t substep;
     int32_t epoch;
+};
+
+// Batched stratified launch: at most kBatchMax problems per launch.
+constexpr int kBatchMax = 256;
+
+struct BatchArgs {
+    const SgdArgs* __restrict__ probs;  // per problem; .epoch = its first epoch, .nblocks = its strata B
+    const int64_t* seg_t0;    // nseg: first ticket of each run of equally wide rounds
+    const int32_t* seg_r0;    // nseg: first round of the run
+    const int32_t* seg_w;     // nseg: tiles per round in the run
+    const int32_t* boff;      // nsorted + 1: offset of each sorted problem's tiles in a round
+    const int32_t* order;     // nsorted: sorted position -> problem index
+    const int64_t* done_off;  // per problem: first of its B progress counters in done
+    uint32_t* done;           // per (problem, row block): rounds completed (uncached)
+    uint64_t* ticket;         // next ticket (uncached)
+    uint32_t* timed_out;      // set when a wave gave up waiting (uncached)
+    int64_t nitems;           // tickets in the launch
+    int32_t nsorted, nseg;
+    uint64_t deadline;        // wall_clock64 ticks a wave may run before it gives up waiting
 };
 
 struct SseArgs {

@@ -24,10 +24,26 @@ __device__ __forceinline__ A step_for(const SgdArgs& a, int32_t count) {
     return A(s);
 }
 
+// Row access of W / H for the stratified kernels: plain (DC = false) when
+// every reader of a row runs after a kernel boundary, device-coherent (the
+// sc1 cache policy of relaxed agent-scope atomics, DC = true) when a row
+// moves between waves on different XCDs inside one launch.  The values are
+// the same.
+template <int S, bool DC>
+__device__ __forceinline__ typename Store<S>::pair row_load(const typename Store<S>::elem* row, int lane,
+                                                            int npairs) {
+    return Store<S>::template load<DC ? kCpDevice : 0>(row, lane, npairs);
+}
+template <int S, bool DC>
+__device__ __forceinline__ void row_store(typename Store<S>::elem* row, int lane, int npairs,
+                                          typename Store<S>::pair p, uint32_t rnd, int round_mode) {
+    Store<S>::template store<DC ? kCpDevice : 0>(row, lane, npairs, p, rnd, round_mode);
+}
+
 // SGD steps for the first `cnt` (<= 64) ratings held one per lane in
 // (myrow, myval), in lane order, for column j whose factor pair h this lane
 // holds.  Up to kPrefetch rows of W are kept in flight.
-template <int S>
+template <int S, bool DC = false>
 __device__ __forceinline__ void run_chunk(const SgdArgs& a, typename Store<S>::pair& h, int32_t myrow,
                                           typename Store<S>::rval myval, int32_t cnt,
                                           typename Store<S>::acc step, typename Store<S>::acc reg,
@@ -42,7 +58,7 @@ __device__ __forceinline__ void run_chunk(const SgdArgs& a, typename Store<S>::p
     typename T::pair w[kPrefetch];
 #pragma unroll
     for (int d = 0; d < kPrefetch; ++d) {
-        w[d] = T::load(W + int64_t(readlane_i32(myrow, min(d, cnt - 1))) * ldf, lane, npairs);
+        w[d] = row_load<S, DC>(W + int64_t(readlane_i32(myrow, min(d, cnt - 1))) * ldf, lane, npairs);
     }
     for (int32_t t0 = 0; t0 < cnt; t0 += kPrefetch) {
 #pragma unroll
@@ -53,12 +69,12 @@ __device__ __forceinline__ void run_chunk(const SgdArgs& a, typename Store<S>::p
                 const Aa r = Aa(readlane_val(myval, t));
                 sgd_step<typename T::pair, Aa>(w[d], h, r, step, reg);
                 const uint32_t rnd = sr_bits(a.sr_seed, uint32_t(i), sr_b, uint32_t(lane));
-                T::store(W + int64_t(i) * ldf, lane, npairs, w[d], rnd, a.round_mode);
+                row_store<S, DC>(W + int64_t(i) * ldf, lane, npairs, w[d], rnd, a.round_mode);
             }
             // refill slot d with the row kPrefetch ratings ahead (clamped: past
             // the end it re-reads a row of this chunk and is unused)
             const int32_t tn = min(t + kPrefetch, cnt - 1);
-            w[d] = T::load(W + int64_t(readlane_i32(myrow, tn)) * ldf, lane, npairs);
+            w[d] = row_load<S, DC>(W + int64_t(readlane_i32(myrow, tn)) * ldf, lane, npairs);
         }
     }
 }
@@ -79,6 +95,28 @@ __device__ __forceinline__ void run_visit(const SgdArgs& a, typename Store<S>::p
         const int32_t myrow = ridx[base + min(lane, cnt - 1)];
         const R myval = rv[base + min(lane, cnt - 1)];
         run_chunk<S>(a, h, myrow, myval, cnt, step, reg, sr_b);
+    }
+}
+
+// run_visit with the first chunk's rows and values already loaded (one per
+// lane, lanes past the visit's length holding the last rating's), so that
+// the caller can fetch them ahead, during the previous visit.
+template <int S, bool DC = false>
+__device__ __forceinline__ void run_visit_pre(const SgdArgs& a, typename Store<S>::pair& h, int32_t j,
+                                              int64_t start, int32_t len, int32_t row0,
+                                              typename Store<S>::rval val0, typename Store<S>::acc step,
+                                              typename Store<S>::acc reg, uint32_t count) {
+    using R = typename Store<S>::rval;
+    const int lane = lane_id();
+    const int32_t* __restrict__ ridx = a.rowidx + start;
+    const R* __restrict__ rv = static_cast<const R*>(a.val) + start;
+    const uint32_t sr_b = uint32_t(j) * 0x01000193u + count;
+    run_chunk<S, DC>(a, h, row0, val0, min(kWave, len), step, reg, sr_b);
+    for (int32_t base = kWave; base < len; base += kWave) {
+        const int32_t cnt = min(kWave, len - base);
+        const int32_t myrow = ridx[base + min(lane, cnt - 1)];
+        const R myval = rv[base + min(lane, cnt - 1)];
+        run_chunk<S, DC>(a, h, myrow, myval, cnt, step, reg, sr_b);
     }
 }
 

@@ -10,6 +10,9 @@
 //
 //   sgd_strat_kernel : one launch per sub-epoch of a B x B stratification;
 //                      wave b owns tile (row block b, column block (b+s) % B).
+//   sgd_batch_kernel : many stratified problems in one persistent launch;
+//                      waves take tiles in ticket order and wait for the two
+//                      tiles of the previous sub-epoch that the tile reads.
 // The asynchronous token kernel is in async_kernel.hip.
 #include <hip/hip_runtime.h>
 
@@ -18,6 +21,8 @@
 #include "sgd_common.hpp"
 
 using namespace nomad_dev;
+using nomad_k::BatchArgs;
+using nomad_k::kBatchMax;
 using nomad_k::SgdArgs;
 using nomad_k::VisitDesc;
 
@@ -27,34 +32,148 @@ using nomad_k::VisitDesc;
 // ascending order, rows of each column in ascending order; the step size uses
 // the epoch as visit count (every (column, row block) pair is visited once per
 // epoch, as with the reference's per-thread col_update_counts at one rank).
-template <int S>
-__global__ __launch_bounds__(64) void sgd_strat_kernel(SgdArgs a) {
+// The tile's non-empty column visits are listed ahead (tile_ptr / visits), so
+// empty (column, row block) pairs cost nothing.  While a visit runs, the next
+// one's descriptor, h row and first 64 rows / values are already being
+// fetched: its column differs from the running one's (a column is visited
+// once per tile), so h is not being written; its W rows may be, and are
+// loaded only when the visit starts.  After the tile's last visit the
+// look-ahead reads the next tile's first visit or the list's sentinel
+// (length 0, start 0), in bounds, and drops it.
+// The single-problem kernel and the batched kernel both run a tile through
+// this function, so a problem's arithmetic is the same in either launch.
+template <int S, bool DC>
+__device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep, int epoch) {
     using T = Store<S>;
     using E = typename T::elem;
     using Aa = typename T::acc;
-    const int b = blockIdx.x;
-    if (b >= a.nblocks) return;
+    using R = typename T::rval;
     const int lane = lane_id();
-    const int cb = (b + a.substep) % a.nblocks;
-    const int32_t rlo = b * a.rpb;
-    const int32_t rhi = min(a.m, rlo + a.rpb);
-    const int32_t clo = cb * a.cpb;
-    const int32_t chi = min(a.n, clo + a.cpb);
-    if (rlo >= rhi || clo >= chi) return;
-    const Aa step = step_for<Aa>(a, a.epoch);
+    const int B = a.nblocks;
+    const int cb = (b + substep) % B;
+    const int64_t* tp = a.tile_ptr + int64_t(b) * B + cb;
+    const int64_t v0 = tp[0], v1 = tp[1];
+    if (v0 >= v1) return;
+    const Aa step = step_for<Aa>(a, epoch);
     const Aa reg = Aa(a.reg);
     E* __restrict__ H = static_cast<E*>(a.H);
-    for (int32_t j = clo; j < chi; ++j) {
-        const int64_t c0 = a.colptr[j], c1 = a.colptr[j + 1];
-        const int64_t s = row_lower_bound(a.rowidx, c0, c1, rlo);
-        const int64_t e = row_lower_bound(a.rowidx, s, c1, rhi);
-        if (e <= s) continue;
-        E* hrow = H + int64_t(j) * a.ldf;
-        typename T::pair h = T::load(hrow, lane, a.ldf >> 1);
-        run_visit<S>(a, h, j, s, int32_t(e - s), step, reg, uint32_t(a.epoch));
+    const R* __restrict__ rv = static_cast<const R*>(a.val);
+    const int npairs = a.ldf >> 1;
+    VisitDesc d = a.visits[v0];
+    typename T::pair h = row_load<S, DC>(H + int64_t(d.worker) * a.ldf, lane, npairs);
+    int64_t fi = d.start + min(lane, max(d.len, 1) - 1);
+    int32_t row = a.rowidx[fi];
+    R val = rv[fi];
+    for (int64_t v = v0; v < v1; ++v) {
+        const VisitDesc nd = a.visits[v + 1];
+        const typename T::pair hn = row_load<S, DC>(H + int64_t(nd.worker) * a.ldf, lane, npairs);
+        const int64_t ni = nd.start + min(lane, max(nd.len, 1) - 1);
+        const int32_t rown = a.rowidx[ni];
+        const R valn = rv[ni];
+        const int32_t j = d.worker;
+        run_visit_pre<S, DC>(a, h, j, d.start, d.len, row, val, step, reg, uint32_t(epoch));
         const uint32_t rnd = sr_bits(a.sr_seed, 0x80000000u | uint32_t(b),
-                                     uint32_t(j) * 0x01000193u + uint32_t(a.epoch), uint32_t(lane));
-        T::store(hrow, lane, a.ldf >> 1, h, rnd, a.round_mode);
+                                     uint32_t(j) * 0x01000193u + uint32_t(epoch), uint32_t(lane));
+        row_store<S, DC>(H + int64_t(j) * a.ldf, lane, npairs, h, rnd, a.round_mode);
+        d = nd;
+        h = hn;
+        row = rown;
+        val = valn;
+    }
+}
+
+template <int S>
+__global__ __launch_bounds__(64) void sgd_strat_kernel(SgdArgs a) {
+    const int b = blockIdx.x;
+    if (b >= a.nblocks) return;
+    strat_tile<S, false>(a, b, a.substep, a.epoch);
+}
+
+// ---------------------------------------------------------------------------
+// Batched stratified sweep.  Problem p (its own ratings, factors, rank, step
+// table and strata count B_p, args.epoch = its first epoch) runs R_p = E * B_p
+// rounds; round r of p is sub-epoch r % B_p of epoch args.epoch + r / B_p,
+// i.e. the B_p tiles (r, b).  Tile (r, b) reads W block b and H block
+// (b + r) % B_p, which the tiles (r - 1, b) and (r - 1, (b + 1) % B_p) wrote
+// last; every tile that touched either block earlier is an ancestor of those
+// two.  So tile (r, b) may start once both have finished, and it then sees
+// exactly the values the one-launch-per-sub-epoch schedule gives it.
+//
+// Tickets: round by round (all problems' tiles of round r, then round r + 1),
+// problems sorted by R_p descending so that the problems still running in a
+// round are a prefix of the sorted list.  A wave takes the next ticket, waits
+// for the two parent tiles (their tickets are smaller, so they were taken by
+// waves that are running), runs the tile and publishes it.  The smallest
+// unfinished ticket can always run, so the launch needs no co-residency and
+// finishes for any grid size.
+//
+// Publication: the next tile on a block may run on another XCD, whose L2 is
+// not this one's.  W and H rows are therefore read and written with
+// device-coherent (sc1) accesses, and a tile's progress counter is raised
+// only after all of the wave's stores have completed (s_waitcnt 0), so a
+// wave that sees the counter reads the new rows; no L2 writeback or
+// invalidate is needed.  Counters live in uncached memory and are polled
+// with device-coherent loads.
+template <int S>
+__global__ __launch_bounds__(256) void sgd_batch_kernel(BatchArgs ba) {
+    __shared__ int64_t seg_t0[kBatchMax];
+    __shared__ int32_t seg_r0[kBatchMax];
+    __shared__ int32_t seg_w[kBatchMax];
+    __shared__ int32_t boff[kBatchMax + 1];
+    __shared__ int32_t order[kBatchMax];
+    const int Q = ba.nsorted, G = ba.nseg;
+    for (int i = threadIdx.x; i <= Q; i += blockDim.x) {
+        if (i < G) {
+            seg_t0[i] = ba.seg_t0[i];
+            seg_r0[i] = ba.seg_r0[i];
+            seg_w[i] = ba.seg_w[i];
+        }
+        boff[i] = ba.boff[i];
+        if (i < Q) order[i] = ba.order[i];
+    }
+    __syncthreads();
+    const int lane = lane_id();
+    const uint64_t t_end = wall_clock64() + ba.deadline;
+    for (;;) {
+        uint64_t t = 0;
+        if (lane == 0) t = __hip_atomic_fetch_add(ba.ticket, uint64_t(1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        t = uniform64(t);
+        if (t >= uint64_t(ba.nitems)) break;
+        // segment: rounds of constant width (the same problems still running)
+        int sg = 0;
+        while (sg + 1 < G && uint64_t(seg_t0[sg + 1]) <= t) ++sg;
+        const uint64_t rel = t - uint64_t(seg_t0[sg]);
+        const int32_t r = seg_r0[sg] + int32_t(rel / uint32_t(seg_w[sg]));
+        const int32_t local = int32_t(rel % uint32_t(seg_w[sg]));
+        int q = 0;
+        while (q + 1 < Q && boff[q + 1] <= local) ++q;
+        const int p = __builtin_amdgcn_readfirstlane(order[q]);
+        const int b = __builtin_amdgcn_readfirstlane(local - boff[q]);
+        const SgdArgs a = ba.probs[p];  // by value: its fields stay in registers across the row stores
+        const int B = a.nblocks;
+        uint32_t* done = ba.done + ba.done_off[p];
+        if (r > 0) {
+            const int b2 = b + 1 == B ? 0 : b + 1;
+            int ok = 1;
+            if (lane == 0) {
+                while (ld_dc(done + b) < uint32_t(r) || ld_dc(done + b2) < uint32_t(r)) {
+                    if (wall_clock64() > t_end) {
+                        ok = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(4);
+                }
+            }
+            ok = __builtin_amdgcn_readfirstlane(ok);
+            if (!ok) {
+                if (lane == 0) st_dc(ba.timed_out, 1u);
+                break;
+            }
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);  // no row access moves above the wait
+        }
+        strat_tile<S, true>(a, b, r % B, a.epoch + r / B);
+        wait_all_mem();
+        if (lane == 0) st_dc(done + b, uint32_t(r + 1));
     }
 }
 
@@ -62,4 +181,7 @@ __global__ __launch_bounds__(64) void sgd_strat_kernel(SgdArgs a) {
 template __global__ void sgd_strat_kernel<NOMAD_STORE_FP32>(SgdArgs);
 template __global__ void sgd_strat_kernel<NOMAD_STORE_BF16>(SgdArgs);
 template __global__ void sgd_strat_kernel<NOMAD_STORE_FP64>(SgdArgs);
+template __global__ void sgd_batch_kernel<NOMAD_STORE_FP32>(BatchArgs);
+template __global__ void sgd_batch_kernel<NOMAD_STORE_BF16>(BatchArgs);
+template __global__ void sgd_batch_kernel<NOMAD_STORE_FP64>(BatchArgs);
 

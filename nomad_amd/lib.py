@@ -71,6 +71,11 @@ class ObjParams(C.Structure):
     _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("k", C.c_int32), ("storage", C.c_int32), ("device", C.c_int32)]
 
 
+class TuneTrial(C.Structure):
+    _fields_ = [("nworkers", C.c_int32), ("flags", C.c_int32), ("status", C.c_int32), ("pad", C.c_int32),
+                ("updates_per_s", C.c_double)]
+
+
 class SynthSpec(C.Structure):
     _fields_ = [
         ("m", C.c_int32), ("n", C.c_int32), ("nnz_train", C.c_int64), ("nnz_test", C.c_int64),
@@ -100,6 +105,9 @@ HIP_SYMBOLS = {
     "nomad_eval": (C.c_int, [_P, C.POINTER(EvalStats)]),
     "nomad_get_nworkers": (C.c_int, [_P, _i32p]),
     "nomad_get_counters": (C.c_int, [_P, _i64p, C.c_int32]),
+    "nomad_train_batch": (C.c_int, [C.POINTER(_P), C.c_int32, C.c_int64, C.c_int32, C.POINTER(TrainStats)]),
+    "nomad_autotune": (C.c_int, [C.POINTER(Params), _i64p, _i32p, _f64p, C.c_int64, C.c_double, C.POINTER(TuneTrial),
+                                 C.POINTER(TuneTrial), C.c_int32, _i32p]),
     "nomad_obj_create": (C.c_int, [C.POINTER(ObjParams), _i64p, _i32p, _f64p, C.c_int64, C.POINTER(_P)]),
     "nomad_obj_eval": (C.c_int, [_P, _P, _P, C.c_double, _P, _P, C.POINTER(C.c_double), _P]),
     "nomad_obj_destroy": (C.c_int, [_P]),
@@ -440,3 +448,41 @@ class Context:
         st = EvalStats()
         self._check(self._lib.nomad_eval(self._h, C.byref(st)))
         return st
+
+
+def train_batch(ctxs: list[Context], epochs: int, waves: int = 0) -> list[TrainStats]:
+    """Runs `epochs` stratified epochs of every context in one persistent
+    launch (nomad_train_batch); each context's result equals its own
+    Context.train(epochs=...) bit for bit."""
+    lib = hip()
+    arr = (_P * len(ctxs))(*[c._h for c in ctxs])
+    stats = (TrainStats * len(ctxs))()
+    rc = lib.nomad_train_batch(arr, len(ctxs), epochs, waves, stats)
+    if rc != 0:
+        raise NomadError(rc, lib.nomad_last_error().decode())
+    return list(stats)
+
+
+def autotune(train: Csc, k: int, *, mode: str = "async", storage: str = "fp32", rounding: str = "nearest",
+             lrate: float = 0.001, drate: float = 0.1, reg: float = 1.0, seed: int = 12345, device: int = 0,
+             trial_seconds: float = 0.05):
+    """nomad_autotune: returns (best TuneTrial, list of all TuneTrials)."""
+    lib = hip()
+    p = Params()
+    p.m, p.n, p.k = train.m, train.n, k
+    p.storage = STORAGE_NAMES[storage]
+    p.mode = {"async": MODE_ASYNC, "strat": MODE_STRAT}[mode]
+    p.rounding = {"nearest": ROUND_NEAREST, "stochastic": ROUND_STOCHASTIC}[rounding]
+    p.device = device
+    p.lrate, p.drate, p.reg, p.seed = lrate, drate, reg, seed
+    colptr = np.ascontiguousarray(train.colptr, np.int64)
+    rowidx = np.ascontiguousarray(train.rowidx, np.int32)
+    val = np.ascontiguousarray(train.val, np.float64)
+    best = TuneTrial()
+    trials = (TuneTrial * 32)()
+    nt = C.c_int32()
+    rc = lib.nomad_autotune(C.byref(p), _ptr(colptr, C.c_int64), _ptr(rowidx, C.c_int32), _ptr(val, C.c_double),
+                            train.nnz, trial_seconds, C.byref(best), trials, 32, C.byref(nt))
+    if rc != 0:
+        raise NomadError(rc, lib.nomad_last_error().decode())
+    return best, list(trials[: min(nt.value, 32)])

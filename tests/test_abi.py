@@ -38,3 +38,19 @@ def test_abi_version_and_error_string():
     L = lib.hip()
     assert L.nomad_abi_version() == 1
     assert isinstance(L.nomad_last_error(), bytes)
+
+
+def test_argument_errors_before_any_device_call():
+    """Argument checks of the batch and autotune entry points return
+    NOMAD_EINVAL with a message, before touching a device."""
+    from nomad_amd import lib
+
+    L = lib.hip()
+    assert L.nomad_train_batch(None, 1, 1, 0, None) == -1
+    arr = (lib._P * 1)(None)
+    assert L.nomad_train_batch(arr, 0, 1, 0, None) == -1
+    assert L.nomad_train_batch(arr, 257, 1, 0, None) == -1
+    assert b"nomad_train_batch" in L.nomad_last_error()
+    best = lib.TuneTrial()
+    assert L.nomad_autotune(None, None, None, None, 0, 0.1, ctypes.byref(best), None, 0, None) == -1
+    assert b"nomad_autotune" in L.nomad_last_error()

@@ -1,0 +1,153 @@
+"""Batched stratified sweep (nomad_train_batch) and the autotuner on the GPU.
+
+The bar for the batch: every problem's factors after a batched launch are
+bit-identical to the same problem trained alone (nomad_train, strat mode),
+for problems of different sizes, ranks, regularisations and strata counts,
+in every storage type, for any persistent grid size, and across calls.
+"""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+STRATA = [3, 8, 16, 5, 1, 32, 12, 7]
+
+
+def _problems(gpu, count, scale, first=0):
+    out = []
+    for p in range(first, first + count):
+        spec, k, reg = gpu.preset_batch(p, scale)
+        tr, te = gpu.synth(spec)
+        out.append((tr, te, k, reg))
+    return out
+
+
+def _ctx(gpu, tr, te, k, reg, B, storage, seed):
+    rounding = "stochastic" if storage == "bf16" else "nearest"
+    c = gpu.Context(tr.m, tr.n, k, storage=storage, rounding=rounding, mode="strat", nworkers=B, seed=seed,
+                    lrate=0.002, drate=0.05, reg=reg)
+    c.upload(tr, te)
+    W0, H0 = gpu.init_factors(tr.m, tr.n, k, 2, 1000 + seed)
+    c.set_factors(W0, H0)
+    return c
+
+
+@pytest.mark.parametrize("storage", ["fp32", "bf16", "fp64"])
+def test_batch_bitwise_equals_single_runs(gpu, storage):
+    probs = _problems(gpu, 6, 0.01)
+    single, batch = [], []
+    for i, (tr, te, k, reg) in enumerate(probs):
+        B = STRATA[i % len(STRATA)]
+        a = _ctx(gpu, tr, te, k, reg, B, storage, seed=i)
+        a.train(epochs=2)
+        a.train(epochs=1)
+        single.append(a)
+        batch.append(_ctx(gpu, tr, te, k, reg, B, storage, seed=i))
+    st = gpu.train_batch(batch, 2)
+    st2 = gpu.train_batch(batch, 1)
+    for i, (a, b) in enumerate(zip(single, batch)):
+        Wa, Ha = a.get_factors()
+        Wb, Hb = b.get_factors()
+        tr = probs[i][0]
+        print(f"[batch {storage}] problem {i}: m={tr.m} n={tr.n} nnz={tr.nnz} k={probs[i][2]} B={a.nworkers}",
+              flush=True)
+        assert np.array_equal(Wa, Wb), f"problem {i}: W differs"
+        assert np.array_equal(Ha, Hb), f"problem {i}: H differs"
+        assert st[i].updates == 2 * tr.nnz and st2[i].updates == tr.nnz
+        assert st2[i].epochs_done == 3
+        assert np.isfinite(Wb).all() and np.abs(Wb).max() > 0
+        a.close()
+        b.close()
+
+
+@pytest.mark.parametrize("waves", [1, 5, 64])
+def test_batch_any_grid_size(gpu, waves):
+    """Few waves (down to one workgroup) must give the same bits: the launch
+    needs no co-residency."""
+    probs = _problems(gpu, 3, 0.01, first=10)
+    ref, got = [], []
+    for i, (tr, te, k, reg) in enumerate(probs):
+        ref.append(_ctx(gpu, tr, te, k, reg, STRATA[i], "fp32", seed=i))
+        got.append(_ctx(gpu, tr, te, k, reg, STRATA[i], "fp32", seed=i))
+    gpu.train_batch(ref, 2)
+    gpu.train_batch(got, 2, waves=waves)
+    for a, b in zip(ref, got):
+        Wa, Ha = a.get_factors()
+        Wb, Hb = b.get_factors()
+        assert np.array_equal(Wa, Wb) and np.array_equal(Ha, Hb)
+        a.close()
+        b.close()
+
+
+def test_batch_sweep_32_problems(gpu):
+    """The sweep shape (32 uneven problems, k in {16..128}, reg in {0.01..0.5},
+    BF16 with stochastic rounding) at 1/100 of its sizes: batched vs one run
+    per problem, bitwise, and the batched launch's time beside the loop's."""
+    probs = _problems(gpu, 32, 0.01)
+    single, batch = [], []
+    t_single = 0.0
+    for i, (tr, te, k, reg) in enumerate(probs):
+        B = 16
+        a = _ctx(gpu, tr, te, k, reg, B, "bf16", seed=i)
+        t_single += a.train(epochs=2).train_seconds
+        single.append(a)
+        batch.append(_ctx(gpu, tr, te, k, reg, B, "bf16", seed=i))
+    t0 = time.perf_counter()
+    st = gpu.train_batch(batch, 2)
+    wall = time.perf_counter() - t0
+    print(f"[batch sweep] 32 problems, {sum(p[0].nnz for p in probs)} ratings: batched {st[0].train_seconds * 1e3:.2f} ms"
+          f" (host wall {wall * 1e3:.1f} ms), one launch sequence per problem {t_single * 1e3:.2f} ms", flush=True)
+    for i, (a, b) in enumerate(zip(single, batch)):
+        Wa, Ha = a.get_factors()
+        Wb, Hb = b.get_factors()
+        assert np.array_equal(Wa, Wb) and np.array_equal(Ha, Hb), f"problem {i} differs"
+        ev = b.eval()
+        assert np.isfinite(ev.test_sse)
+        a.close()
+        b.close()
+
+
+def test_batch_rejects_mixed_contexts(gpu, tiny):
+    tr, te = tiny
+    a = gpu.Context(tr.m, tr.n, 8, mode="strat", nworkers=4)
+    b = gpu.Context(tr.m, tr.n, 8, mode="async", nworkers=64)
+    c = gpu.Context(tr.m, tr.n, 8, storage="fp64", mode="strat", nworkers=4)
+    for x in (a, b, c):
+        x.upload(tr, te)
+    for bad in ([a, b], [a, c], [a, a]):
+        with pytest.raises(gpu.NomadError) as e:
+            gpu.train_batch(bad, 1)
+        assert e.value.code == -1
+    d = gpu.Context(tr.m, tr.n, 8, mode="strat", nworkers=4)  # no data uploaded
+    with pytest.raises(gpu.NomadError) as e:
+        gpu.train_batch([a, d], 1)
+    assert e.value.code == -3
+    for x in (a, b, c, d):
+        x.close()
+
+
+@pytest.mark.parametrize("mode", ["async", "strat"])
+def test_autotune_picks_a_valid_fastest_config(gpu, small, mode):
+    tr, te = small
+    best, trials = gpu.autotune(tr, 16, mode=mode, storage="fp32", lrate=0.001, drate=0.1, reg=0.05,
+                                trial_seconds=0.02)
+    ok = [t for t in trials if t.status == 0]
+    print(f"[autotune {mode}] " + ", ".join(f"({t.nworkers},{t.flags}):{t.updates_per_s / 1e9:.3f}G" for t in trials),
+          flush=True)
+    assert ok and best.status == 0
+    assert best.updates_per_s == max(t.updates_per_s for t in ok)
+    assert any(t.nworkers == best.nworkers and t.flags == best.flags for t in ok)
+    # the chosen configuration trains
+    c = gpu.Context(tr.m, tr.n, 16, mode=mode, nworkers=best.nworkers, balance=bool(best.flags & gpu.FLAG_BALANCE),
+                    lrate=0.001, drate=0.1, reg=0.05)
+    assert c.nworkers == best.nworkers
+    c.upload(tr, te)
+    W0, H0 = gpu.init_factors(tr.m, tr.n, 16, 4, 12345)
+    c.set_factors(W0, H0)
+    st = c.train(epochs=2)
+    assert st.updates >= tr.nnz
+    ev = c.eval()
+    assert np.isfinite(ev.test_sse)
+    c.close()
