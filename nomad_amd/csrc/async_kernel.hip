@@ -66,7 +66,8 @@ struct AsyncShared {
     using R = typename Store<S>::rval;
     static constexpr int kPbw = (NW + 4) & ~3;
     static constexpr int kVd = stream_depth<S, NW>();
-    A vic[NW][kVd][2 * kWave];   // per wave: the last kVd updated rows of its stream
+    typename Store<S>::raw vic[NW][kVd][kWave];  // per wave: the last kVd updated rows of its stream,
+                                                 // as stored
     A h[SLOTS][2 * kWave];       // column factors of resident tokens
     int32_t wrow[SLOTS][kWave];  // window of the column's ratings here: rows
     R wval[SLOTS][kWave];        //                                    values
@@ -146,7 +147,7 @@ __device__ __forceinline__ void tok_store(TokInfo* p, uint32_t start, int32_t le
 
 }  // namespace
 
-template <int S, int NW, int SLOTS>
+template <int S, int NW, int SLOTS, int RM>
 __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
     using T = Store<S>;
     using E = typename T::elem;
@@ -167,6 +168,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
     const int lane = lane_id();
     const int npairs = a.ldf >> 1;
     const int64_t ldf = a.ldf;
+    const uint32_t lkey = sr_lane_key(a.sr_seed, uint32_t(lane));
     const int32_t* wb_g = a.wbound + int64_t(g) * NW;
     const int32_t row_lo = __builtin_amdgcn_readfirstlane(wb_g[wave]);
     const int32_t row_hi = __builtin_amdgcn_readfirstlane(wb_g[wave + 1]);
@@ -290,8 +292,15 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                 }
 
                 const uint64_t t_c0 = clk();
-                P wp[kPf];
-                Aa* const vring = &sh.vic[wave][0][0];
+                typename T::raw wp[kPf];
+                typename T::raw* const vring = &sh.vic[wave][0][0];
+                // stochastic-rounding hash of each stream rating (lane u: its row
+                // and its piece's update key), one vector pass for the stream
+                uint32_t s_rh = 0;
+                if constexpr (RM == NOMAD_ROUND_STOCHASTIC) {
+                    const uint32_t srb_u = uint32_t(__shfl(int32_t(t_srb), my_t));
+                    s_rh = mix32(a.sr_seed ^ mix32(uint32_t(srow) * 0x9E3779B1u + srb_u));
+                }
                 // Each prologue load is preceded by a store that the range check
                 // drops, so the prologue issues memory operations in the same
                 // pattern as the loop and the compiler's wait counts stay as
@@ -300,8 +309,8 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                 for (int d = 0; d < kPf; ++d) {
                     const int32_t r0 = readlane_i32(srow, min(d, total - 1));
                     // (distinct offsets: stores to one offset would be merged away)
-                    T::store_b(wrs, kDrop - d * 1024, 0, P{Aa(0), Aa(0)}, 0u, a.round_mode);
-                    wp[d] = T::load_b(wrs, woff(r0));
+                    T::store_b(wrs, kDrop - d * 1024, 0, P{Aa(0), Aa(0)}, 0u, RM);
+                    wp[d] = T::load_raw_b(wrs, woff(r0));
                 }
                 int cur_t = 0;
                 int s_cur = readlane_i32(ts, 0);
@@ -315,7 +324,6 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                     hn.y = sh.h[s1][2 * lane + 1];
                 }
                 Aa step = readlane_val(t_step, 0);
-                uint32_t srb = uint32_t(readlane_i32(int32_t(t_srb), 0));
                 // kPf ratings per iteration in groups of kGroup.  A group that
                 // starts past the stream leaves both loops at once (an exit
                 // edge, so the loop keeps exact wait counts); slots of the last
@@ -340,7 +348,6 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                                 cur_t = t;
                                 s_cur = readlane_i32(ts, t);
                                 step = readlane_val(t_step, t);
-                                srb = uint32_t(readlane_i32(int32_t(t_srb), t));
                                 if (t + 1 < nt) {
                                     const int sn = readlane_i32(ts, t + 1);
                                     hn.x = sh.h[sn][2 * lane];
@@ -352,28 +359,22 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                             // wp[d] would first wait for wp[d]'s load
                             const int pd = readlane_i32(pdist, uc);
                             P fw = P{Aa(0), Aa(0)};
-                            if (pd != 0) {
-                                const Aa* src = vring + ((d - pd + kPf) % kPf) * (2 * kWave);
-                                fw.x = src[2 * lane];
-                                fw.y = src[2 * lane + 1];
-                            }
+                            if (pd != 0) fw = T::unpack(vring[((d - pd + kPf) % kPf) * kWave + lane]);
+                            const P wl = T::unpack(wp[d]);
                             P w;
-                            w.x = pd != 0 ? fw.x : wp[d].x;
-                            w.y = pd != 0 ? fw.y : wp[d].y;
+                            w.x = pd != 0 ? fw.x : wl.x;
+                            w.y = pd != 0 ? fw.y : wl.y;
                             const P h0 = h;
                             sgd_step<P, Aa>(w, h, Aa(readlane_val(sval, uc)), step, reg);
                             h.x = real ? h.x : h0.x;
                             h.y = real ? h.y : h0.y;
-                            const uint32_t rnd = sr_bits(a.sr_seed, uint32_t(i), srb, uint32_t(lane));
-                            T::store_b(wrs, real ? woff(i) : kDrop, 0, w, rnd, a.round_mode);
-                            {
-                                const P wr = T::rounded(w, rnd, a.round_mode);
-                                Aa* dst = vring + d * (2 * kWave);
-                                dst[2 * lane] = wr.x;
-                                dst[2 * lane + 1] = wr.y;
-                            }
+                            // = sr_bits_k(a.sr_seed, i, key of piece t, lkey)
+                            const uint32_t rnd = uint32_t(readlane_i32(int32_t(s_rh), uc)) ^ lkey;
+                            const typename T::raw wr = T::pack(w, rnd, RM);
+                            T::store_raw_b(wrs, real ? woff(i) : kDrop, wr);
+                            vring[d * kWave + lane] = wr;
                             const int un = min(u + kPf, total - 1);
-                            wp[d] = T::load_b(wrs, woff(readlane_i32(srow, un)));
+                            wp[d] = T::load_raw_b(wrs, woff(readlane_i32(srow, un)));
                         }
                     }
                 }
@@ -422,9 +423,9 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                         P hx;
                         hx.x = sh.h[s][2 * lane];
                         hx.y = sh.h[s][2 * lane + 1];
-                        const uint32_t rnd = sr_bits(a.sr_seed, 0x80000000u | uint32_t(g),
-                                                     uint32_t(readlane_i32(int32_t(t_srb), t)), uint32_t(lane));
-                        T::store_dc(H + int64_t(j) * ldf, lane, npairs, hx, rnd, a.round_mode);
+                        const uint32_t rnd = sr_bits_k(a.sr_seed, 0x80000000u | uint32_t(g),
+                                                     uint32_t(readlane_i32(int32_t(t_srb), t)), lkey);
+                        T::store_dc(H + int64_t(j) * ldf, lane, npairs, hx, rnd, RM);
                         if (lane == 0) {
                             tok_store(a.tok + j, sh.nstart[s], sh.nlen[s], readlane_i32(t_np, t),
                                       readlane_i32(t_ncyc, t));
@@ -491,14 +492,22 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
             // rows past the piece (another wave's, or the end of the range) are
             // never updated; their prefetch slots re-read the piece's first row
             const int32_t r_first = readlane_i32(rA, 0);
-            P wp[kPfL];
+            typename T::raw wp[kPfL];
 #pragma unroll
             for (int d = 0; d < kPfL; ++d) {
                 int32_t rr = readlane_i32(rA, d);
                 rr = rr < row_hi ? rr : r_first;
-                T::store_b(wrs, kDrop - d * 1024, 0, P{Aa(0), Aa(0)}, 0u, a.round_mode);  // dropped: see the stream prologue
-                wp[d] = T::load_b(wrs, woff(rr));
+                T::store_b(wrs, kDrop - d * 1024, 0, P{Aa(0), Aa(0)}, 0u, RM);  // dropped: see the stream prologue
+                wp[d] = T::load_raw_b(wrs, woff(rr));
             }
+            // stochastic-rounding hash of each chunk-A rating (as in the stream)
+            auto chunk_hash = [&](int32_t rows) -> uint32_t {
+                if constexpr (RM == NOMAD_ROUND_STOCHASTIC)
+                    return mix32(a.sr_seed ^ mix32(uint32_t(rows) * 0x9E3779B1u + sr_b));
+                else
+                    return 0u;
+            };
+            uint32_t rhA = chunk_hash(rA);
             // piece ratings in chunk A: rows ascend, so they are a prefix
             int32_t cntA = __popcll(__ballot(rA < row_hi));
             int32_t cnt = 0;
@@ -512,17 +521,19 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                         const int uc = real ? uu : cntA - 1;
                         const int32_t i = readlane_i32(rA, uc);
                         const P h0 = h;
-                        sgd_step<P, Aa>(wp[d], h, Aa(readlane_val(vA, uc)), step, reg);
+                        P w = T::unpack(wp[d]);
+                        sgd_step<P, Aa>(w, h, Aa(readlane_val(vA, uc)), step, reg);
                         h.x = real ? h.x : h0.x;
                         h.y = real ? h.y : h0.y;
-                        const uint32_t rnd = sr_bits(a.sr_seed, uint32_t(i), sr_b, uint32_t(lane));
-                        T::store_b(wrs, real ? woff(i) : kDrop, 0, wp[d], rnd, a.round_mode);
+                        // = sr_bits_k(a.sr_seed, i, sr_b, lkey)
+                        const uint32_t rnd = uint32_t(readlane_i32(int32_t(rhA), uc)) ^ lkey;
+                        T::store_b(wrs, real ? woff(i) : kDrop, 0, w, rnd, RM);
                         const int tg = uu + kPfL;  // < 2 * kWave
                         const int32_t ra = readlane_i32(rA, tg & (kWave - 1));
                         const int32_t rb = readlane_i32(rB, tg & (kWave - 1));
                         int32_t rn = tg < kWave ? ra : rb;
                         rn = rn < row_hi ? rn : i;
-                        wp[d] = T::load_b(wrs, woff(rn));
+                        wp[d] = T::load_raw_b(wrs, woff(rn));
                     }
                 }
                 cnt += cntA;
@@ -530,6 +541,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                 // the piece goes on in the next chunk
                 rA = rB;
                 vA = vB;
+                rhA = chunk_hash(rA);
                 c0 += kWave;
                 rB = INT_MAX;
                 vB = R(0);
@@ -578,8 +590,8 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                 const int32_t np = sh.npos[s];
                 const int32_t ncyc = cycle + (np == 0 ? 1 : 0);
                 const bool park = np == 0 && ncyc >= a.target_cycle;
-                const uint32_t rnd = sr_bits(a.sr_seed, 0x80000000u | uint32_t(g), sr_b, uint32_t(lane));
-                T::store_dc(H + int64_t(j) * ldf, lane, npairs, h, rnd, a.round_mode);
+                const uint32_t rnd = sr_bits_k(a.sr_seed, 0x80000000u | uint32_t(g), sr_b, lkey);
+                T::store_dc(H + int64_t(j) * ldf, lane, npairs, h, rnd, RM);
                 if (lane == 0) tok_store(a.tok + j, sh.nstart[s], sh.nlen[s], np, ncyc);
                 if (park) {
                     wait_all_mem();
@@ -769,9 +781,9 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
         P h;
         h.x = sh.h[s][2 * lane];
         h.y = sh.h[s][2 * lane + 1];
-        const uint32_t rnd = sr_bits(a.sr_seed, 0x80000000u | uint32_t(g),
-                                     uint32_t(j) * 0x01000193u + uint32_t(sh.cyc[s]), uint32_t(lane));
-        T::store_dc(H + int64_t(j) * ldf, lane, npairs, h, rnd, a.round_mode);
+        const uint32_t rnd = sr_bits_k(a.sr_seed, 0x80000000u | uint32_t(g),
+                                     uint32_t(j) * 0x01000193u + uint32_t(sh.cyc[s]), lkey);
+        T::store_dc(H + int64_t(j) * ldf, lane, npairs, h, rnd, RM);
     }
     // lanes hold their own sums; fold them into lane 0
     for (int off = 32; off > 0; off >>= 1) {
@@ -803,10 +815,13 @@ __global__ void seed_rings_kernel(SgdArgs a, const int32_t* list, const int64_t*
     a.qtail[int64_t(p) * nomad_k::kQStride] = t;
 }
 
-#define NOMAD_ASYNC_INST(NW)                                                      \
-    template __global__ void sgd_async_kernel<NOMAD_STORE_FP32, NW, 64>(SgdArgs); \
-    template __global__ void sgd_async_kernel<NOMAD_STORE_BF16, NW, 64>(SgdArgs); \
-    template __global__ void sgd_async_kernel<NOMAD_STORE_FP64, NW, 32>(SgdArgs);
+// The rounding of bf16 stores is a template argument, so the per-rating
+// store carries no branch on it; fp32 and fp64 ignore it.
+#define NOMAD_ASYNC_INST(NW)                                                                              \
+    template __global__ void sgd_async_kernel<NOMAD_STORE_FP32, NW, 64, NOMAD_ROUND_NEAREST>(SgdArgs);    \
+    template __global__ void sgd_async_kernel<NOMAD_STORE_BF16, NW, 64, NOMAD_ROUND_NEAREST>(SgdArgs);    \
+    template __global__ void sgd_async_kernel<NOMAD_STORE_BF16, NW, 64, NOMAD_ROUND_STOCHASTIC>(SgdArgs); \
+    template __global__ void sgd_async_kernel<NOMAD_STORE_FP64, NW, 32, NOMAD_ROUND_NEAREST>(SgdArgs);
 NOMAD_ASYNC_INST(4)
 NOMAD_ASYNC_INST(8)
 NOMAD_ASYNC_INST(16)

@@ -31,7 +31,7 @@ template <int S>
 __global__ void sgd_strat_kernel(SgdArgs);
 template <int S>
 __global__ void sgd_batch_kernel(nomad_k::BatchArgs);
-template <int S, int NW, int SLOTS>
+template <int S, int NW, int SLOTS, int RM>
 __global__ void sgd_async_kernel(SgdArgs);
 template <int S>
 __global__ void sse_kernel(SseArgs);
@@ -569,26 +569,50 @@ constexpr int async_slots() {
     return S == NOMAD_STORE_FP64 ? 32 : 64;
 }
 
-template <int S>
-hipError_t launch_async(const SgdArgs& a, int nw, hipStream_t st) {
+// async kernel instances: rounding is a template argument for bf16 only
+template <int S, int RM>
+hipError_t launch_async_rm(const SgdArgs& a, int nw, hipStream_t st) {
     constexpr int slots = async_slots<S>();
     switch (nw) {
-        case 4: hipLaunchKernelGGL((sgd_async_kernel<S, 4, slots>), dim3(a.nblocks), dim3(4 * 64), 0, st, a); break;
-        case 8: hipLaunchKernelGGL((sgd_async_kernel<S, 8, slots>), dim3(a.nblocks), dim3(8 * 64), 0, st, a); break;
-        default: hipLaunchKernelGGL((sgd_async_kernel<S, 16, slots>), dim3(a.nblocks), dim3(16 * 64), 0, st, a);
+        case 4:
+            hipLaunchKernelGGL((sgd_async_kernel<S, 4, slots, RM>), dim3(a.nblocks), dim3(4 * 64), 0, st, a);
+            break;
+        case 8:
+            hipLaunchKernelGGL((sgd_async_kernel<S, 8, slots, RM>), dim3(a.nblocks), dim3(8 * 64), 0, st, a);
+            break;
+        default: hipLaunchKernelGGL((sgd_async_kernel<S, 16, slots, RM>), dim3(a.nblocks), dim3(16 * 64), 0, st, a);
     }
     return hipGetLastError();
 }
 
 template <int S>
-hipError_t async_occupancy(int nw, int* per_cu) {
+hipError_t launch_async(const SgdArgs& a, int nw, int rounding, hipStream_t st) {
+    if constexpr (S == NOMAD_STORE_BF16) {
+        if (rounding == NOMAD_ROUND_STOCHASTIC) return launch_async_rm<S, NOMAD_ROUND_STOCHASTIC>(a, nw, st);
+    }
+    return launch_async_rm<S, NOMAD_ROUND_NEAREST>(a, nw, st);
+}
+
+template <int S, int RM>
+hipError_t async_occupancy_rm(int nw, int* per_cu) {
     constexpr int slots = async_slots<S>();
     switch (nw) {
-        case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, sgd_async_kernel<S, 4, slots>, 4 * 64, 0);
-        case 8: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, sgd_async_kernel<S, 8, slots>, 8 * 64, 0);
+        case 4:
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, sgd_async_kernel<S, 4, slots, RM>, 4 * 64, 0);
+        case 8:
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, sgd_async_kernel<S, 8, slots, RM>, 8 * 64, 0);
         default:
-            return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, sgd_async_kernel<S, 16, slots>, 16 * 64, 0);
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, sgd_async_kernel<S, 16, slots, RM>, 16 * 64,
+                                                                0);
     }
+}
+
+template <int S>
+hipError_t async_occupancy(int nw, int rounding, int* per_cu) {
+    if constexpr (S == NOMAD_STORE_BF16) {
+        if (rounding == NOMAD_ROUND_STOCHASTIC) return async_occupancy_rm<S, NOMAD_ROUND_STOCHASTIC>(nw, per_cu);
+    }
+    return async_occupancy_rm<S, NOMAD_ROUND_NEAREST>(nw, per_cu);
 }
 
 template <int S>
@@ -679,7 +703,7 @@ int train_async(nomad_ctx* c, double seconds, int64_t epochs, nomad_train_stats*
     if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
     if (e == hipSuccess)
         e = dispatch_storage(c->p.storage,
-                             [&](auto s) { return launch_async<decltype(s)::value>(a, c->nw, c->stream); });
+                             [&](auto s) { return launch_async<decltype(s)::value>(a, c->nw, c->p.rounding, c->stream); });
     if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     cleanup();
@@ -1023,7 +1047,7 @@ int nomad_create(const nomad_params* p, nomad_ctx** out) {
                 break;
             }
         hipError_t oe = dispatch_storage(p->storage, [&](auto t) {
-            return async_occupancy<decltype(t)::value>(nw, &per_cu);
+            return async_occupancy<decltype(t)::value>(nw, p->rounding, &per_cu);
         });
         if (oe != hipSuccess || per_cu < 1) return bail(fail(NOMAD_EHIP, "nomad_create: occupancy query failed"));
         const int gmax = cus * per_cu;

@@ -94,8 +94,19 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
     return x;
 }
 
+// Stochastic-rounding bits of element pair `lane` of row (or column) a at
+// update key b: a hash of (seed, a, b), which is the same for the whole wave
+// and runs on the scalar unit, XOR a per-lane key fixed for the wave.  Each
+// lane's bits are as uniform as the hash; the vector cost per store is one
+// XOR.
+__device__ __forceinline__ uint32_t sr_lane_key(uint32_t seed, uint32_t lane) {
+    return mix32(seed ^ ((lane + 1u) * 0x85EBCA6Bu));
+}
+__device__ __forceinline__ uint32_t sr_bits_k(uint32_t seed, uint32_t a, uint32_t b, uint32_t lkey) {
+    return mix32(seed ^ mix32(a * 0x9E3779B1u + b)) ^ lkey;
+}
 __device__ __forceinline__ uint32_t sr_bits(uint32_t seed, uint32_t a, uint32_t b, uint32_t lane) {
-    return mix32(seed ^ mix32(a * 0x9E3779B1u + b) ^ (lane * 0x85EBCA6Bu));
+    return sr_bits_k(seed, a, b, sr_lane_key(seed, lane));
 }
 
 // ---- storage traits: load/store one factor row as per-lane pairs ----
@@ -174,6 +185,15 @@ struct Store<NOMAD_STORE_FP32> {
         const u32x2_t v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
         return pair{__uint_as_float(v.x), __uint_as_float(v.y)};
     }
+    // a loaded pair as it sits in registers before use (see Store<BF16>)
+    using raw = pair;
+    static __device__ __forceinline__ raw load_raw_b(__amdgpu_buffer_rsrc_t r, int voff) { return load_b(r, voff); }
+    static __device__ __forceinline__ pair unpack(raw v) { return v; }
+    // the stored form of a pair (rounding for bf16) and its store
+    static __device__ __forceinline__ raw pack(pair p, uint32_t, int) { return p; }
+    static __device__ __forceinline__ void store_raw_b(__amdgpu_buffer_rsrc_t r, int voff, raw v) {
+        store_b(r, voff, 0, v, 0u, 0);
+    }
     static __device__ __forceinline__ void store_b(__amdgpu_buffer_rsrc_t r, int voff, int soff, pair p, uint32_t,
                                                    int) {
         const u32x2_t v = {__float_as_uint(p.x), __float_as_uint(p.y)};
@@ -223,6 +243,14 @@ struct Store<NOMAD_STORE_FP64> {
         const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
         return pair{__builtin_bit_cast(double, (uint64_t(v.y) << 32) | v.x),
                     __builtin_bit_cast(double, (uint64_t(v.w) << 32) | v.z)};
+    }
+    using raw = pair;
+    static __device__ __forceinline__ raw load_raw_b(__amdgpu_buffer_rsrc_t r, int voff) { return load_b(r, voff); }
+    static __device__ __forceinline__ pair unpack(raw v) { return v; }
+    // the stored form of a pair (rounding for bf16) and its store
+    static __device__ __forceinline__ raw pack(pair p, uint32_t, int) { return p; }
+    static __device__ __forceinline__ void store_raw_b(__amdgpu_buffer_rsrc_t r, int voff, raw v) {
+        store_b(r, voff, 0, v, 0u, 0);
     }
     static __device__ __forceinline__ void store_b(__amdgpu_buffer_rsrc_t r, int voff, int soff, pair p, uint32_t,
                                                    int) {
@@ -277,6 +305,28 @@ struct Store<NOMAD_STORE_BF16> {
     static __device__ __forceinline__ pair load_b(__amdgpu_buffer_rsrc_t r, int voff, int soff = 0) {
         const uint32_t u = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
         return pair{bf16_lo(u), bf16_hi(u)};
+    }
+    // Prefetched rows stay packed until used: widening right after the load
+    // would make the compiler wait for the load there, which defeats the
+    // prefetch.
+    using raw = uint32_t;
+    static __device__ __forceinline__ raw load_raw_b(__amdgpu_buffer_rsrc_t r, int voff) {
+        return __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0);
+    }
+    static __device__ __forceinline__ pair unpack(raw u) { return pair{bf16_lo(u), bf16_hi(u)}; }
+    static __device__ __forceinline__ raw pack(pair p, uint32_t rnd, int round_mode) {
+        uint32_t lo, hi;
+        if (round_mode == NOMAD_ROUND_STOCHASTIC) {
+            lo = f32_to_bf16_sr(p.x, rnd);
+            hi = f32_to_bf16_sr(p.y, rnd >> 16);
+        } else {
+            lo = f32_to_bf16_rne(p.x);
+            hi = f32_to_bf16_rne(p.y);
+        }
+        return lo | (hi << 16);
+    }
+    static __device__ __forceinline__ void store_raw_b(__amdgpu_buffer_rsrc_t r, int voff, raw v) {
+        __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, 0, 0);
     }
     static __device__ __forceinline__ void store_b(__amdgpu_buffer_rsrc_t r, int voff, int soff, pair p,
                                                    uint32_t rnd, int round_mode) {

@@ -55,6 +55,7 @@ __device__ __forceinline__ void run_chunk(const SgdArgs& a, typename Store<S>::p
     E* __restrict__ W = static_cast<E*>(a.W);
     const int npairs = a.ldf >> 1;
     const int64_t ldf = a.ldf;
+    const uint32_t lkey = sr_lane_key(a.sr_seed, uint32_t(lane));
     typename T::pair w[kPrefetch];
 #pragma unroll
     for (int d = 0; d < kPrefetch; ++d) {
@@ -68,7 +69,7 @@ __device__ __forceinline__ void run_chunk(const SgdArgs& a, typename Store<S>::p
                 const int32_t i = readlane_i32(myrow, t);
                 const Aa r = Aa(readlane_val(myval, t));
                 sgd_step<typename T::pair, Aa>(w[d], h, r, step, reg);
-                const uint32_t rnd = sr_bits(a.sr_seed, uint32_t(i), sr_b, uint32_t(lane));
+                const uint32_t rnd = sr_bits_k(a.sr_seed, uint32_t(i), sr_b, lkey);
                 row_store<S, DC>(W + int64_t(i) * ldf, lane, npairs, w[d], rnd, a.round_mode);
             }
             // refill slot d with the row kPrefetch ratings ahead (clamped: past

@@ -73,8 +73,10 @@ uint32_t mix32(uint32_t x) {
     x ^= x >> 16;
     return x;
 }
+// stochastic-rounding bits: hash of (seed, a, b) XOR a per-lane key
 uint32_t sr_bits(uint32_t seed, uint32_t a, uint32_t b, uint32_t lane) {
-    return mix32(seed ^ mix32(a * 0x9E3779B1u + b) ^ (lane * 0x85EBCA6Bu));
+    const uint32_t lane_key = mix32(seed ^ ((lane + 1u) * 0x85EBCA6Bu));
+    return mix32(seed ^ mix32(a * 0x9E3779B1u + b)) ^ lane_key;
 }
 
 // ---------------------------------------------------------------------------
