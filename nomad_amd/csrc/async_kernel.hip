@@ -323,7 +323,10 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                     hn.x = sh.h[s1][2 * lane];
                     hn.y = sh.h[s1][2 * lane + 1];
                 }
-                Aa step = readlane_val(t_step, 0);
+                // per piece: c1 = 1 - step * reg and -step (the step's two coefficients)
+                const Aa t_c1 = Aa(1) - t_step * reg;
+                Aa c1 = readlane_val(t_c1, 0);
+                Aa nstep = -readlane_val(t_step, 0);
                 // kPf ratings per iteration in groups of kGroup.  A group that
                 // starts past the stream leaves both loops at once (an exit
                 // edge, so the loop keeps exact wait counts); slots of the last
@@ -347,7 +350,8 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                                 h = hn;
                                 cur_t = t;
                                 s_cur = readlane_i32(ts, t);
-                                step = readlane_val(t_step, t);
+                                c1 = readlane_val(t_c1, t);
+                                nstep = -readlane_val(t_step, t);
                                 if (t + 1 < nt) {
                                     const int sn = readlane_i32(ts, t + 1);
                                     hn.x = sh.h[sn][2 * lane];
@@ -364,10 +368,8 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                             P w;
                             w.x = pd != 0 ? fw.x : wl.x;
                             w.y = pd != 0 ? fw.y : wl.y;
-                            const P h0 = h;
-                            sgd_step<P, Aa>(w, h, Aa(readlane_val(sval, uc)), step, reg);
-                            h.x = real ? h.x : h0.x;
-                            h.y = real ? h.y : h0.y;
+                            // idle slots: c1 = 1, step 0 leave h as it is
+                            sgd_step_c<P, Aa>(w, h, Aa(readlane_val(sval, uc)), real ? c1 : Aa(1), real ? nstep : Aa(0));
                             // = sr_bits_k(a.sr_seed, i, key of piece t, lkey)
                             const uint32_t rnd = uint32_t(readlane_i32(int32_t(s_rh), uc)) ^ lkey;
                             const typename T::raw wr = T::pack(w, rnd, RM);
@@ -468,6 +470,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
             const int32_t cycle = sh.cyc[s];
             const Aa step = sh.stp[s];
             const uint32_t sr_b = uint32_t(j) * 0x01000193u + uint32_t(cycle);
+            const Aa c1 = Aa(1) - step * reg;
             P h;
             h.x = sh.h[s][2 * lane];
             h.y = sh.h[s][2 * lane + 1];
@@ -520,11 +523,8 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                         const bool real = uu < cntA;
                         const int uc = real ? uu : cntA - 1;
                         const int32_t i = readlane_i32(rA, uc);
-                        const P h0 = h;
                         P w = T::unpack(wp[d]);
-                        sgd_step<P, Aa>(w, h, Aa(readlane_val(vA, uc)), step, reg);
-                        h.x = real ? h.x : h0.x;
-                        h.y = real ? h.y : h0.y;
+                        sgd_step_c<P, Aa>(w, h, Aa(readlane_val(vA, uc)), real ? c1 : Aa(1), real ? -step : Aa(0));
                         // = sr_bits_k(a.sr_seed, i, sr_b, lkey)
                         const uint32_t rnd = uint32_t(readlane_i32(int32_t(rhA), uc)) ^ lkey;
                         T::store_b(wrs, real ? woff(i) : kDrop, 0, w, rnd, RM);
