@@ -93,6 +93,40 @@ def run_path(lib, tr, te, W0, H0, *, storage, rounding, k, hyp, nworkers, warmup
                 t_hit=t_hit, nworkers=nw)
 
 
+def profiled_traffic(storage: str) -> float | None:
+    """HBM (L2-to-fabric) bytes per update of the async kernel measured by the
+    two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE, each scaled by a
+    plain copy of known size) summarised in profiles/; None if absent."""
+    import re
+
+    f = ROOT / "profiles" / "sgd_async_kernel_netflix_k100_pmc.md"
+    if not f.exists():
+        return None
+    m = re.search(rf"^traffic_bytes_per_update\[{storage}\]: (\S+)$", f.read_text(), flags=re.M)
+    return float(m.group(1)) if m else None
+
+
+def copy_calibrate(nbytes: int) -> None:
+    """One hipMemcpy device-to-device of nbytes, which the HIP runtime runs as
+    a blit kernel reading and writing nbytes: under rocprofv3 --pmc its
+    FETCH_SIZE / WRITE_SIZE give the counters' scale (KiB per byte moved)."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"copy calibration: {nbytes} bytes device to device")
+
+
 def cpu_baseline(tr, W0, H0, k, hyp, seconds):
     from oracle import oracle as O
 
@@ -207,6 +241,9 @@ def main() -> int:
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp32", "bf16"], default=None)
     ap.add_argument("--workload", choices=["netflix", "sweep"], default="netflix")
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
+                    help="first run one device-to-device copy of BYTES through the HIP runtime (a plain copy "
+                         "kernel of known size, the reference for the HBM traffic counters)")
     ap.add_argument("--problems", type=int, default=32, help="sweep: number of problems")
     ap.add_argument("--strata", default="", help="sweep: comma-separated strata per problem (cycled) instead of "
                                                    "the autotuner's choice")
@@ -219,6 +256,8 @@ def main() -> int:
     from nomad_amd import lib
 
     lib.hip()
+    if a.copy_calibrate > 0:
+        copy_calibrate(a.copy_calibrate)
     if a.workload == "sweep":
         print(json.dumps(sweep(a, lib)), flush=True)
         return 0
@@ -245,6 +284,8 @@ def main() -> int:
     head = res[head_name]
     value = head["updates"] / head["dev_s"]
     achieved = head["alg_bytes"] / head["dev_s"] / 1e9
+    tpu = profiled_traffic(head_name)
+    traffic = tpu * head["updates"] / a.steps if tpu is not None else None
     out = {
         "metric": "updates/s + time-to-target test RMSE, Netflix-shape k=100, BF16 vs FP32, 1 GPU",
         "value": value,
@@ -266,7 +307,9 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per update (copy-calibrated), "
+                              "profiles/sgd_async_kernel_netflix_k100_pmc.md, x updates per launch",
             "kernel": f"sgd_async_kernel<{head_name}>",
             "alg_bytes_per_launch": head["alg_bytes"] / a.steps,
             "avg_launch_ms": head["dev_s"] / a.steps * 1e3,
