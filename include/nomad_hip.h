@@ -72,8 +72,10 @@ typedef struct nomad_params {
     int32_t rounding;  /* NOMAD_ROUND_* (bf16 only)                              */
     int32_t nworkers;  /* async: workers = row blocks (0: 8 per compute unit);
                           strat: strata B (0: one per co-resident wave)          */
-    int32_t ptoken;    /* tokens a worker may hold ahead of the one it is
-                          processing (README --ptoken); 0 = default            */
+    int32_t ptoken;    /* README --ptoken (tokens in the pipeline; the option is
+                          commented out in nomad_option.cpp:43-44).  Accepted,
+                          >= 0; the async kernel's token capacity is its LDS
+                          slots per workgroup (64, 32 for fp64), not this   */
     int32_t device;    /* HIP device ordinal                                     */
     int32_t flags;     /* NOMAD_FLAG_* bits; others must be 0                    */
     double lrate;      /* --lrate  */

@@ -44,6 +44,7 @@ struct Options {
     int dim = 100;
     bool pause = true;
     std::string input, output;
+    int ptoken = 0;
     int verbose = 0;
     // GPU
     int workers = 0, device = 0;
@@ -130,6 +131,9 @@ bool parse_args(int argc, char** argv, Options& o) {
         {"random_send", 0, "accepted, no effect at one rank", Spec::kIgnoredValue, &ignored_d},
         {"r0delay", 0, "accepted, no effect at one rank", Spec::kIgnoredValue, &ignored_d},
         {"input", 'i', "path of data (directory with train.dat and test.dat)", Spec::kString, &o.input},
+        {"path", 0, "the same as --input (README spelling)", Spec::kString, &o.input},
+        {"ptoken", 0, "README's tokens in the pipeline: accepted, >= 0 (the GPU holds 64 per workgroup)",
+         Spec::kInt, &o.ptoken},
         {"output", 'o', "path prefix of the factor dump (<output>0)", Spec::kString, &o.output},
         {"recover", 0, "accepted, no effect", Spec::kIgnoredValue, &ignored_d},
         {"jobid", 'j', "accepted, no effect", Spec::kIgnoredValue, &ignored_d},
@@ -253,7 +257,7 @@ bool parse_args(int argc, char** argv, Options& o) {
                 bad = true;
                 break;
             }
-        if (o.dim < 1 || o.dim > 128 || o.nthreads < 0) bad = true;
+        if (o.dim < 1 || o.dim > 128 || o.nthreads < 0 || o.ptoken < 0) bad = true;
     }
     if (bad || help) {
         usage(specs);
@@ -307,6 +311,7 @@ int main(int argc, char** argv) {
     p.mode = o.mode;
     p.rounding = o.rounding;
     p.nworkers = o.workers;
+    p.ptoken = o.ptoken;
     p.device = o.device;
     p.flags = (o.balance && o.mode == NOMAD_MODE_ASYNC) ? NOMAD_FLAG_BALANCE : 0;
     p.lrate = o.lrate;

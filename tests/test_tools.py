@@ -129,3 +129,8 @@ def test_driver_rejects_bad_options(tmp_path):
     assert r.returncode != 0
     r = run_driver("-i", str(tmp_path), "--timeout", "1")  # no train.dat there
     assert r.returncode != 0 and "train.dat" in r.stderr
+    # README spellings: --path for the data, --ptoken accepted (non-negative)
+    r = run_driver("--path", str(tmp_path), "--ptoken", "1024", "--timeout", "1")
+    assert r.returncode != 0 and "train.dat" in r.stderr
+    r = run_driver("--path", str(tmp_path), "--ptoken", "-1")  # rejected before any file is read
+    assert r.returncode != 0 and "cannot open" not in r.stderr
