@@ -27,7 +27,7 @@ using nomad_k::SseArgs;
 using nomad_k::TokInfo;
 using nomad_k::VisitDesc;
 
-template <int S>
+template <int S, int RM>
 __global__ void sgd_strat_kernel(SgdArgs);
 template <int S>
 __global__ void sgd_batch_kernel(nomad_k::BatchArgs);
@@ -328,6 +328,9 @@ int upload_csc(nomad_ctx* c, DevCsc& d, const int64_t* colptr, const int32_t* ro
 int ensure_tiles(nomad_ctx* c) {
     if (c->tiles_ready) return NOMAD_OK;
     const int32_t n = c->p.n, B = c->P;
+    // a row block is addressed through one buffer resource (32-bit range)
+    if (int64_t(c->rpb) * c->ldf * int64_t(c->esz) >= (int64_t(1) << 31))
+        return fail(NOMAD_EINVAL, "strat mode: a row block exceeds 2 GiB of W; use more strata");
     std::vector<std::vector<VisitDesc>> per(static_cast<size_t>(B));
     for (int32_t j = 0; j < n; ++j) {
         const int64_t c0 = c->h_colptr[j], c1 = c->h_colptr[j + 1];
@@ -549,7 +552,13 @@ SgdArgs base_args(nomad_ctx* c) {
 
 template <int S>
 hipError_t launch_strat(const SgdArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(sgd_strat_kernel<S>, dim3(a.nblocks), dim3(64), 0, st, a);
+    if constexpr (S == NOMAD_STORE_BF16) {
+        if (a.round_mode == NOMAD_ROUND_STOCHASTIC) {
+            hipLaunchKernelGGL((sgd_strat_kernel<S, NOMAD_ROUND_STOCHASTIC>), dim3(a.nblocks), dim3(64), 0, st, a);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((sgd_strat_kernel<S, NOMAD_ROUND_NEAREST>), dim3(a.nblocks), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 

@@ -187,12 +187,18 @@ struct Store<NOMAD_STORE_FP32> {
     }
     // a loaded pair as it sits in registers before use (see Store<BF16>)
     using raw = pair;
-    static __device__ __forceinline__ raw load_raw_b(__amdgpu_buffer_rsrc_t r, int voff) { return load_b(r, voff); }
+    template <int CP = 0>
+    static __device__ __forceinline__ raw load_raw_b(__amdgpu_buffer_rsrc_t r, int voff) {
+        const u32x2_t v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, CP);
+        return pair{__uint_as_float(v.x), __uint_as_float(v.y)};
+    }
     static __device__ __forceinline__ pair unpack(raw v) { return v; }
     // the stored form of a pair (rounding for bf16) and its store
     static __device__ __forceinline__ raw pack(pair p, uint32_t, int) { return p; }
-    static __device__ __forceinline__ void store_raw_b(__amdgpu_buffer_rsrc_t r, int voff, raw v) {
-        store_b(r, voff, 0, v, 0u, 0);
+    template <int CP = 0>
+    static __device__ __forceinline__ void store_raw_b(__amdgpu_buffer_rsrc_t r, int voff, raw p) {
+        const u32x2_t v = {__float_as_uint(p.x), __float_as_uint(p.y)};
+        __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, 0, CP);
     }
     static __device__ __forceinline__ void store_b(__amdgpu_buffer_rsrc_t r, int voff, int soff, pair p, uint32_t,
                                                    int) {
@@ -245,12 +251,20 @@ struct Store<NOMAD_STORE_FP64> {
                     __builtin_bit_cast(double, (uint64_t(v.w) << 32) | v.z)};
     }
     using raw = pair;
-    static __device__ __forceinline__ raw load_raw_b(__amdgpu_buffer_rsrc_t r, int voff) { return load_b(r, voff); }
+    template <int CP = 0>
+    static __device__ __forceinline__ raw load_raw_b(__amdgpu_buffer_rsrc_t r, int voff) {
+        const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, CP);
+        return pair{__builtin_bit_cast(double, (uint64_t(v.y) << 32) | v.x),
+                    __builtin_bit_cast(double, (uint64_t(v.w) << 32) | v.z)};
+    }
     static __device__ __forceinline__ pair unpack(raw v) { return v; }
     // the stored form of a pair (rounding for bf16) and its store
     static __device__ __forceinline__ raw pack(pair p, uint32_t, int) { return p; }
-    static __device__ __forceinline__ void store_raw_b(__amdgpu_buffer_rsrc_t r, int voff, raw v) {
-        store_b(r, voff, 0, v, 0u, 0);
+    template <int CP = 0>
+    static __device__ __forceinline__ void store_raw_b(__amdgpu_buffer_rsrc_t r, int voff, raw p) {
+        const uint64_t a = __builtin_bit_cast(uint64_t, p.x), b = __builtin_bit_cast(uint64_t, p.y);
+        const u32x4_t v = {uint32_t(a), uint32_t(a >> 32), uint32_t(b), uint32_t(b >> 32)};
+        __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, 0, CP);
     }
     static __device__ __forceinline__ void store_b(__amdgpu_buffer_rsrc_t r, int voff, int soff, pair p, uint32_t,
                                                    int) {
@@ -310,8 +324,9 @@ struct Store<NOMAD_STORE_BF16> {
     // would make the compiler wait for the load there, which defeats the
     // prefetch.
     using raw = uint32_t;
+    template <int CP = 0>
     static __device__ __forceinline__ raw load_raw_b(__amdgpu_buffer_rsrc_t r, int voff) {
-        return __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0);
+        return __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, CP);
     }
     static __device__ __forceinline__ pair unpack(raw u) { return pair{bf16_lo(u), bf16_hi(u)}; }
     static __device__ __forceinline__ raw pack(pair p, uint32_t rnd, int round_mode) {
@@ -325,8 +340,9 @@ struct Store<NOMAD_STORE_BF16> {
         }
         return lo | (hi << 16);
     }
+    template <int CP = 0>
     static __device__ __forceinline__ void store_raw_b(__amdgpu_buffer_rsrc_t r, int voff, raw v) {
-        __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, 0, CP);
     }
     static __device__ __forceinline__ void store_b(__amdgpu_buffer_rsrc_t r, int voff, int soff, pair p,
                                                    uint32_t rnd, int round_mode) {

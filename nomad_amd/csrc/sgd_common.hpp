@@ -14,8 +14,6 @@ namespace nomad_dev {
 using nomad_k::SgdArgs;
 using nomad_k::VisitDesc;
 
-constexpr int kPrefetch = 8;  // rows of W in flight per worker inside a visit
-
 template <class A>
 __device__ __forceinline__ A step_for(const SgdArgs& a, int32_t count) {
     if (count < a.nsteps) return static_cast<const A*>(a.steps)[count];
@@ -40,98 +38,113 @@ __device__ __forceinline__ void row_store(typename Store<S>::elem* row, int lane
     Store<S>::template store<DC ? kCpDevice : 0>(row, lane, npairs, p, rnd, round_mode);
 }
 
-// SGD steps for the first `cnt` (<= 64) ratings held one per lane in
-// (myrow, myval), in lane order, for column j whose factor pair h this lane
-// holds.  Up to kPrefetch rows of W are kept in flight.
-template <int S, bool DC = false>
-__device__ __forceinline__ void run_chunk(const SgdArgs& a, typename Store<S>::pair& h, int32_t myrow,
-                                          typename Store<S>::rval myval, int32_t cnt,
-                                          typename Store<S>::acc step, typename Store<S>::acc reg,
-                                          uint32_t sr_b) {
+// W rows of one row block [rlo, rlo + rows) through one buffer resource:
+// lane l's pair of row r sits at byte offset wvoff + (r - rlo) * rowbytes,
+// wvoff = l * pair bytes (or 2^31 for lanes past the row, so that they are
+// out of range), and kDrop puts every lane out of range (a store the range
+// check drops).
+struct RowBlock {
+    __amdgpu_buffer_rsrc_t rsrc;
+    uint32_t wvoff;
+    int32_t rlo;
+    int32_t rowbytes;
+    __device__ __forceinline__ int off(int32_t r) const { return int(wvoff + uint32_t(r - rlo) * uint32_t(rowbytes)); }
+    __device__ __forceinline__ int drop(int d = 0) const { return int(wvoff + 0x7FFFFFFFu) - d * 1024; }
+};
+
+template <int S>
+__device__ __forceinline__ RowBlock row_block(const SgdArgs& a, int32_t rlo, int32_t rhi, int lane) {
     using T = Store<S>;
-    using E = typename T::elem;
+    RowBlock rb;
+    rb.rowbytes = a.ldf * int(sizeof(typename T::elem));
+    rb.rsrc = __builtin_amdgcn_make_buffer_rsrc(static_cast<typename T::elem*>(a.W) + int64_t(rlo) * a.ldf, 0,
+                                                (rhi - rlo) * rb.rowbytes, 0x00020000);
+    rb.wvoff = lane < (a.ldf >> 1) ? uint32_t(lane * T::kPairBytes) : 0x80000000u;
+    rb.rlo = rlo;
+    return rb;
+}
+
+// rows of W in flight inside a visit of the stratified kernels; the batched
+// kernel's device-coherent loads miss the L2, so it keeps fewer in flight
+// (past a visit's end the prefetch slots re-read its last row)
+template <int S, bool DC>
+constexpr int strat_depth() {
+    return (S == NOMAD_STORE_FP64 || DC) ? 8 : 16;
+}
+
+// SGD steps for the first `cnt` (<= 64) ratings held one per lane in
+// (myrow, myval), in lane order, for the column whose factor pair h this lane
+// holds; the rows lie in row block rb.  strat_depth rows of W are in flight.
+// The loop body has no branch around a memory operation (slots past cnt are
+// idle: their store is dropped by the range check and h keeps its value) and
+// the prologue issues a dropped store before each load, so the compiler's
+// waits are exactly the prefetch distance.  A column's rows are distinct, so
+// no prefetched row is updated before its rating uses it.
+template <int S, bool DC, int RM>
+__device__ __forceinline__ void run_chunk(const SgdArgs& a, const RowBlock& rb, typename Store<S>::pair& h,
+                                          int32_t myrow, typename Store<S>::rval myval, int32_t cnt,
+                                          typename Store<S>::acc step, typename Store<S>::acc reg, uint32_t sr_b,
+                                          uint32_t lkey) {
+    using T = Store<S>;
     using Aa = typename T::acc;
-    const int lane = lane_id();
-    E* __restrict__ W = static_cast<E*>(a.W);
-    const int npairs = a.ldf >> 1;
-    const int64_t ldf = a.ldf;
-    const uint32_t lkey = sr_lane_key(a.sr_seed, uint32_t(lane));
-    typename T::pair w[kPrefetch];
+    using P = typename T::pair;
+    constexpr int kD = strat_depth<S, DC>();
+    constexpr int CP = DC ? kCpDevice : 0;
+    typename T::raw wp[kD];
 #pragma unroll
-    for (int d = 0; d < kPrefetch; ++d) {
-        w[d] = row_load<S, DC>(W + int64_t(readlane_i32(myrow, min(d, cnt - 1))) * ldf, lane, npairs);
+    for (int d = 0; d < kD; ++d) {
+        T::template store_raw_b<CP>(rb.rsrc, rb.drop(d), T::pack(P{Aa(0), Aa(0)}, 0u, 0));
+        wp[d] = T::template load_raw_b<CP>(rb.rsrc, rb.off(readlane_i32(myrow, min(d, cnt - 1))));
     }
-    for (int32_t t0 = 0; t0 < cnt; t0 += kPrefetch) {
+    // groups of kG ratings; a group that starts past the chunk leaves both
+    // loops at once (an exit edge keeps the loop's wait counts exact)
+    constexpr int kG = 4;
+    for (int32_t t0 = 0;; t0 += kD) {
 #pragma unroll
-        for (int d = 0; d < kPrefetch; ++d) {
-            const int32_t t = t0 + d;
-            if (t < cnt) {
-                const int32_t i = readlane_i32(myrow, t);
-                const Aa r = Aa(readlane_val(myval, t));
-                sgd_step<typename T::pair, Aa>(w[d], h, r, step, reg);
+        for (int gq = 0; gq < kD / kG; ++gq) {
+            if (t0 + gq * kG >= cnt) goto chunk_done;
+#pragma unroll
+            for (int q = 0; q < kG; ++q) {
+                const int d = gq * kG + q;
+                const int32_t t = t0 + d;
+                const bool real = t < cnt;
+                const int32_t uc = real ? t : cnt - 1;
+                const int32_t i = readlane_i32(myrow, uc);
+                P w = T::unpack(wp[d]);
+                const P h0 = h;
+                sgd_step<P, Aa>(w, h, Aa(readlane_val(myval, uc)), step, reg);
+                h.x = real ? h.x : h0.x;
+                h.y = real ? h.y : h0.y;
                 const uint32_t rnd = sr_bits_k(a.sr_seed, uint32_t(i), sr_b, lkey);
-                row_store<S, DC>(W + int64_t(i) * ldf, lane, npairs, w[d], rnd, a.round_mode);
+                T::template store_raw_b<CP>(rb.rsrc, real ? rb.off(i) : rb.drop(), T::pack(w, rnd, RM));
+                wp[d] = T::template load_raw_b<CP>(rb.rsrc, rb.off(readlane_i32(myrow, min(t + kD, cnt - 1))));
             }
-            // refill slot d with the row kPrefetch ratings ahead (clamped: past
-            // the end it re-reads a row of this chunk and is unused)
-            const int32_t tn = min(t + kPrefetch, cnt - 1);
-            w[d] = row_load<S, DC>(W + int64_t(readlane_i32(myrow, tn)) * ldf, lane, npairs);
         }
     }
+chunk_done:;
 }
 
-// Walks ratings [start, start + len) of column j for one worker.  h is the
-// column's factor pair held by this lane; it is updated in place.
-template <int S>
-__device__ __forceinline__ void run_visit(const SgdArgs& a, typename Store<S>::pair& h, int32_t j,
-                                          int64_t start, int32_t len, typename Store<S>::acc step,
-                                          typename Store<S>::acc reg, uint32_t count) {
-    using R = typename Store<S>::rval;
-    const int lane = lane_id();
-    const int32_t* __restrict__ ridx = a.rowidx + start;
-    const R* __restrict__ rv = static_cast<const R*>(a.val) + start;
-    const uint32_t sr_b = uint32_t(j) * 0x01000193u + count;
-    for (int32_t base = 0; base < len; base += kWave) {
-        const int32_t cnt = min(kWave, len - base);
-        const int32_t myrow = ridx[base + min(lane, cnt - 1)];
-        const R myval = rv[base + min(lane, cnt - 1)];
-        run_chunk<S>(a, h, myrow, myval, cnt, step, reg, sr_b);
-    }
-}
-
-// run_visit with the first chunk's rows and values already loaded (one per
-// lane, lanes past the visit's length holding the last rating's), so that
-// the caller can fetch them ahead, during the previous visit.
-template <int S, bool DC = false>
-__device__ __forceinline__ void run_visit_pre(const SgdArgs& a, typename Store<S>::pair& h, int32_t j,
-                                              int64_t start, int32_t len, int32_t row0,
+// The ratings [start, start + len) of column j in row block rb, with the
+// first chunk's rows and values already loaded (one per lane, lanes past the
+// visit's length holding the last rating's), so that the caller can fetch
+// them ahead, during the previous visit.
+template <int S, bool DC, int RM>
+__device__ __forceinline__ void run_visit_pre(const SgdArgs& a, const RowBlock& rb, typename Store<S>::pair& h,
+                                              int32_t j, int64_t start, int32_t len, int32_t row0,
                                               typename Store<S>::rval val0, typename Store<S>::acc step,
-                                              typename Store<S>::acc reg, uint32_t count) {
+                                              typename Store<S>::acc reg, uint32_t count, uint32_t lkey) {
     using R = typename Store<S>::rval;
     const int lane = lane_id();
     const int32_t* __restrict__ ridx = a.rowidx + start;
     const R* __restrict__ rv = static_cast<const R*>(a.val) + start;
     const uint32_t sr_b = uint32_t(j) * 0x01000193u + count;
-    run_chunk<S, DC>(a, h, row0, val0, min(kWave, len), step, reg, sr_b);
+    run_chunk<S, DC, RM>(a, rb, h, row0, val0, min(kWave, len), step, reg, sr_b, lkey);
     for (int32_t base = kWave; base < len; base += kWave) {
         const int32_t cnt = min(kWave, len - base);
         const int32_t myrow = ridx[base + min(lane, cnt - 1)];
         const R myval = rv[base + min(lane, cnt - 1)];
-        run_chunk<S, DC>(a, h, myrow, myval, cnt, step, reg, sr_b);
+        run_chunk<S, DC, RM>(a, rb, h, myrow, myval, cnt, step, reg, sr_b, lkey);
     }
-}
-
-// lower_bound of `key` in the ascending rows rowidx[lo, hi)
-__device__ __forceinline__ int64_t row_lower_bound(const int32_t* rowidx, int64_t lo, int64_t hi,
-                                                   int32_t key) {
-    while (lo < hi) {
-        const int64_t mid = lo + ((hi - lo) >> 1);
-        if (rowidx[mid] < key)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
 }
 
 __device__ __forceinline__ uint64_t uniform64(uint64_t v) {

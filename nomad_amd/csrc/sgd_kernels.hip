@@ -42,7 +42,7 @@ using nomad_k::VisitDesc;
 // (length 0, start 0), in bounds, and drops it.
 // The single-problem kernel and the batched kernel both run a tile through
 // this function, so a problem's arithmetic is the same in either launch.
-template <int S, bool DC>
+template <int S, bool DC, int RM>
 __device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep, int epoch) {
     using T = Store<S>;
     using E = typename T::elem;
@@ -59,6 +59,8 @@ __device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep,
     E* __restrict__ H = static_cast<E*>(a.H);
     const R* __restrict__ rv = static_cast<const R*>(a.val);
     const int npairs = a.ldf >> 1;
+    const RowBlock rb = row_block<S>(a, b * a.rpb, min(a.m, (b + 1) * a.rpb), lane);
+    const uint32_t lkey = sr_lane_key(a.sr_seed, uint32_t(lane));
     VisitDesc d = a.visits[v0];
     typename T::pair h = row_load<S, DC>(H + int64_t(d.worker) * a.ldf, lane, npairs);
     int64_t fi = d.start + min(lane, max(d.len, 1) - 1);
@@ -71,10 +73,10 @@ __device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep,
         const int32_t rown = a.rowidx[ni];
         const R valn = rv[ni];
         const int32_t j = d.worker;
-        run_visit_pre<S, DC>(a, h, j, d.start, d.len, row, val, step, reg, uint32_t(epoch));
-        const uint32_t rnd = sr_bits(a.sr_seed, 0x80000000u | uint32_t(b),
-                                     uint32_t(j) * 0x01000193u + uint32_t(epoch), uint32_t(lane));
-        row_store<S, DC>(H + int64_t(j) * a.ldf, lane, npairs, h, rnd, a.round_mode);
+        run_visit_pre<S, DC, RM>(a, rb, h, j, d.start, d.len, row, val, step, reg, uint32_t(epoch), lkey);
+        const uint32_t rnd = sr_bits_k(a.sr_seed, 0x80000000u | uint32_t(b),
+                                       uint32_t(j) * 0x01000193u + uint32_t(epoch), lkey);
+        row_store<S, DC>(H + int64_t(j) * a.ldf, lane, npairs, h, rnd, RM);
         d = nd;
         h = hn;
         row = rown;
@@ -82,11 +84,13 @@ __device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep,
     }
 }
 
-template <int S>
+// (RM: the rounding of bf16 stores, a template argument so that no store
+// branches on it; fp32 and fp64 ignore it)
+template <int S, int RM>
 __global__ __launch_bounds__(64) void sgd_strat_kernel(SgdArgs a) {
     const int b = blockIdx.x;
     if (b >= a.nblocks) return;
-    strat_tile<S, false>(a, b, a.substep, a.epoch);
+    strat_tile<S, false, RM>(a, b, a.substep, a.epoch);
 }
 
 // ---------------------------------------------------------------------------
@@ -171,16 +175,24 @@ __global__ __launch_bounds__(256) void sgd_batch_kernel(BatchArgs ba) {
             }
             __atomic_signal_fence(__ATOMIC_SEQ_CST);  // no row access moves above the wait
         }
-        strat_tile<S, true>(a, b, r % B, a.epoch + r / B);
+        if constexpr (S == NOMAD_STORE_BF16) {  // problems may round differently
+            if (a.round_mode == NOMAD_ROUND_STOCHASTIC)
+                strat_tile<S, true, NOMAD_ROUND_STOCHASTIC>(a, b, r % B, a.epoch + r / B);
+            else
+                strat_tile<S, true, NOMAD_ROUND_NEAREST>(a, b, r % B, a.epoch + r / B);
+        } else {
+            strat_tile<S, true, NOMAD_ROUND_NEAREST>(a, b, r % B, a.epoch + r / B);
+        }
         wait_all_mem();
         if (lane == 0) st_dc(done + b, uint32_t(r + 1));
     }
 }
 
 // Explicit instantiations launched from capi.hip.
-template __global__ void sgd_strat_kernel<NOMAD_STORE_FP32>(SgdArgs);
-template __global__ void sgd_strat_kernel<NOMAD_STORE_BF16>(SgdArgs);
-template __global__ void sgd_strat_kernel<NOMAD_STORE_FP64>(SgdArgs);
+template __global__ void sgd_strat_kernel<NOMAD_STORE_FP32, NOMAD_ROUND_NEAREST>(SgdArgs);
+template __global__ void sgd_strat_kernel<NOMAD_STORE_BF16, NOMAD_ROUND_NEAREST>(SgdArgs);
+template __global__ void sgd_strat_kernel<NOMAD_STORE_BF16, NOMAD_ROUND_STOCHASTIC>(SgdArgs);
+template __global__ void sgd_strat_kernel<NOMAD_STORE_FP64, NOMAD_ROUND_NEAREST>(SgdArgs);
 template __global__ void sgd_batch_kernel<NOMAD_STORE_FP32>(BatchArgs);
 template __global__ void sgd_batch_kernel<NOMAD_STORE_BF16>(BatchArgs);
 template __global__ void sgd_batch_kernel<NOMAD_STORE_FP64>(BatchArgs);
