@@ -21,7 +21,9 @@ separate context that is then discarded.
 
 Prints one JSON line.
 
-Second workload (not the default): `--workload sweep` runs the batched
+Other workloads (not the default line): `--workload ccd` (CCD++ rank-one
+sweeps at the Netflix shape), `--workload objective` (objective and gradient
+at 100,000 x 20,000 with 10 M cells, k = 64), and `--workload sweep`: the batched
 hyper-parameter sweep -- 32 independent problems of uneven sizes (users ~
 logU(10k, 200k), items ~ logU(2k, 50k), ratings ~ logU(0.5M, 20M), k in
 {16, 32, 64, 128}, reg in {0.01, 0.05, 0.1, 0.5}), BF16 storage with
@@ -151,16 +153,13 @@ def sweep(a, lib) -> dict:
     log(f"sweep: generated {len(probs)} problems, {total} training ratings in {time.perf_counter() - t0:.1f} s")
     strata, tune_s = [], time.perf_counter()
     fixed = [int(x) for x in a.strata.split(",")] if a.strata else []
-    for i, (tr, te, k, reg) in enumerate(probs):
-        if fixed:
-            strata.append(fixed[i % len(fixed)])
-            continue
-        best, trials = lib.autotune(tr, k, mode="strat", storage="bf16", rounding="stochastic", reg=reg, seed=12345,
-                                    **hyp)
-        strata.append(best.nworkers)
-        log(f"sweep: problem {i} m={tr.m} n={tr.n} nnz={tr.nnz} k={k} reg={reg}: B={best.nworkers} "
-            f"({best.updates_per_s / 1e9:.3f} G/s; trials " +
-            ", ".join(f"{t.nworkers}:{t.updates_per_s / 1e9:.3f}" for t in trials) + ")")
+    if fixed:
+        strata = [fixed[i % len(fixed)] for i in range(len(probs))]
+    else:
+        # the batch is tuned as a whole: the problems share the device
+        strata, rate = lib.autotune_batch([(tr, k, reg) for tr, te, k, reg in probs], storage="bf16",
+                                          rounding="stochastic", **hyp)
+        log(f"sweep: autotuned strata {sorted(set(strata))} ({rate / 1e9:.3f} G updates/s in the trial)")
     tune_s = time.perf_counter() - tune_s
 
     def contexts():
@@ -229,6 +228,75 @@ def sweep(a, lib) -> dict:
     }
 
 
+def ccd_workload(a, lib) -> dict:
+    """CCD++ rank-one sweeps (nomad_ccd) on the Netflix shape: one outer
+    iteration over all k ranks, a.ccd_inner inner iterations per rank."""
+    k = a.ccd_rank
+    tr, te = lib.synth(lib.preset(2, a.scale))
+    W0, _ = lib.init_factors(tr.m, tr.n, k, 1, 12345)
+    H0 = np.zeros((tr.n, k))
+    _, _, obj, rmse, st = lib.ccd(tr, W0, H0, 0.05, 1, a.ccd_inner, eps=1e-3, test=te)
+    log(f"ccd: {st.ranks_done} ranks, {st.inner_done} inner iterations in {st.seconds:.3f} s, "
+        f"final obj {obj[-1]:.6g} test RMSE {rmse[-1]:.6f}")
+    gbs = st.alg_bytes / st.seconds / 1e9
+    return {"metric": "CCD++ rank-one sweep, Netflix-shape, FP64, 1 GPU", "value": st.inner_done / st.seconds,
+            "unit": "inner iterations/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
+            "data": "synthetic", "seconds": st.seconds,
+            "config": {"workload": f"ccd++ k={k}, 1 outer iteration, {a.ccd_inner} inner, lambda 0.05",
+                       "m": tr.m, "n": tr.n, "nnz_train": tr.nnz},
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                         "note": "wall time of the call (host set-up included)"},
+            "final_test_rmse": float(rmse[-1])}
+
+
+def objective_workload(a, lib) -> dict:
+    """Objective + full gradient (nomad_obj_eval) at the gradient-check shape
+    (100,000 x 20,000, 10 M cells, k = 64), device arrays, a.steps evaluations."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    tr, _ = lib.synth(lib.preset(4, a.scale))
+    k = 64
+    out = {}
+    for storage, esz in (("fp64", 8), ("fp32", 4)):
+        ob = lib.Objective(tr, k, storage=storage)
+        ldf = (k + 1) & ~1
+        rng = np.random.default_rng(1)
+        Wh = (rng.random((tr.m, ldf)) / 8).astype(np.float64 if esz == 8 else np.float32)
+        Hh = (rng.random((tr.n, ldf)) / 8).astype(Wh.dtype)
+        ptrs = []
+        for arr in (Wh, Hh, Wh, Hh):
+            d = C.c_void_p()
+            if hip.hipMalloc(C.byref(d), C.c_size_t(arr.nbytes)) != 0:
+                raise RuntimeError("hipMalloc failed")
+            hip.hipMemcpy(d, arr.ctypes.data_as(C.c_void_p), C.c_size_t(arr.nbytes), 1)
+            ptrs.append(d.value)
+        ob.eval(ptrs[0], ptrs[1], 0.05, ptrs[2], ptrs[3])  # warm-up
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            L = ob.eval(ptrs[0], ptrs[1], 0.05, ptrs[2], ptrs[3])
+        dt = (time.perf_counter() - t0) / a.steps
+        for d in ptrs:
+            hip.hipFree(C.c_void_p(d))
+        ob.close()
+        # per rating: row index + value, w_i and h_j read in each of the two
+        # passes, plus the gradients written once per line element
+        alg = 2.0 * tr.nnz * (12 + 2 * k * esz) + 2.0 * (tr.m + tr.n) * k * esz
+        out[storage] = {"ms_per_eval": dt * 1e3, "loss": L, "alg_GBps": alg / dt / 1e9}
+        log(f"objective {storage}: {dt * 1e3:.3f} ms per evaluation (loss {L:.6g})")
+    gbs = out["fp64"]["alg_GBps"]
+    return {"metric": "objective + gradient evaluations/s, 100k x 20k x 10M, k=64, FP64, 1 GPU",
+            "value": 1e3 / out["fp64"]["ms_per_eval"], "unit": "evaluations/s", "n_gpus": 1,
+            "higher_is_better": True, "dtype": "f64", "data": "synthetic", "steps": a.steps,
+            "config": {"workload": "objective and full gradient, device arrays", "m": tr.m, "n": tr.n,
+                       "nnz": tr.nnz, "k": k},
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                         "note": "host wall time per evaluation (launches + one loss read back)"},
+            "paths": out}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,7 +308,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp32", "bf16"], default=None)
-    ap.add_argument("--workload", choices=["netflix", "sweep"], default="netflix")
+    ap.add_argument("--workload", choices=["netflix", "sweep", "ccd", "objective"], default="netflix")
+    ap.add_argument("--ccd-rank", type=int, default=100)
+    ap.add_argument("--ccd-inner", type=int, default=3)
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
                     help="first run one device-to-device copy of BYTES through the HIP runtime (a plain copy "
                          "kernel of known size, the reference for the HBM traffic counters)")
@@ -258,8 +328,9 @@ def main() -> int:
     lib.hip()
     if a.copy_calibrate > 0:
         copy_calibrate(a.copy_calibrate)
-    if a.workload == "sweep":
-        print(json.dumps(sweep(a, lib)), flush=True)
+    if a.workload != "netflix":
+        fn = {"sweep": sweep, "ccd": ccd_workload, "objective": objective_workload}[a.workload]
+        print(json.dumps(fn(a, lib)), flush=True)
         return 0
     k = 100
     hyp = dict(lrate=0.012, drate=0.05, reg=0.05)

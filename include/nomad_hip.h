@@ -173,6 +173,27 @@ int nomad_autotune(const nomad_params* params, const int64_t* colptr, const int3
                    int64_t nnz, double trial_seconds, nomad_tune_trial* best, nomad_tune_trial* trials,
                    int32_t max_trials, int32_t* ntrials);
 
+/* Batch autotuner (no reference counterpart): picks the strata count for a
+ * batched sweep by timing whole batched epochs (nomad_train_batch), since
+ * the problems then share the device and the best count differs from each
+ * problem's own optimum.  Candidates: one strata count for every problem
+ * in {64, 128, 256, 512, 1024} (each capped at the problem's rows).  Each
+ * trial builds scratch strat contexts of the problems (params as given,
+ * nworkers replaced; factors drawn from mt19937_64(params.seed), U(0,
+ * 1/sqrt(k))) and times one batched epoch after a warm-up epoch.
+ * strata_out[i] receives problem i's chosen count, *updates_per_s the
+ * batch's rate with it.  count <= 256. */
+typedef struct nomad_batch_problem {
+    nomad_params params;  /* NOMAD_MODE_STRAT; one storage type and device for all */
+    const int64_t* colptr;
+    const int32_t* rowidx;
+    const double* val;
+    int64_t nnz;
+} nomad_batch_problem;
+
+int nomad_autotune_batch(const nomad_batch_problem* probs, int32_t count, int32_t* strata_out,
+                         double* updates_per_s);
+
 /* Sum of squared errors over the training and test ratings, accumulated in
  * FP64 (the dot product runs in the storage's arithmetic type). */
 int nomad_eval(nomad_ctx* ctx, nomad_eval_stats* stats);

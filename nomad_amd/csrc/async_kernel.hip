@@ -272,6 +272,21 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                     srow = sh.wrow[ms][mo];
                     sval = sh.wval[ms][mo];
                 }
+                // The stream's first W rows are requested as soon as the rows are
+                // known; the rest of the set-up runs while they arrive.  Each
+                // prologue load is preceded by a store that the range check
+                // drops, so the prologue issues memory operations in the same
+                // pattern as the loop and the compiler's wait counts stay as
+                // loose as the prefetch distance allows.
+                const uint64_t t_c0 = clk();
+                typename T::raw wp[kPf];
+#pragma unroll
+                for (int d = 0; d < kPf; ++d) {
+                    const int32_t r0 = readlane_i32(srow, min(d, total - 1));
+                    // (distinct offsets: stores to one offset would be merged away)
+                    T::store_b(wrs, kDrop - d * 1024, 0, P{Aa(0), Aa(0)}, 0u, RM);
+                    wp[d] = T::load_raw_b(wrs, woff(r0));
+                }
                 Aa t_step = Aa(0);
                 uint32_t t_srb = 0;
                 int32_t t_cyc = 0;
@@ -290,9 +305,6 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                     const int32_t o = __shfl_up(srow, b);
                     if (lane >= b && lane < total && o == srow) pdist = b;
                 }
-
-                const uint64_t t_c0 = clk();
-                typename T::raw wp[kPf];
                 typename T::raw* const vring = &sh.vic[wave][0][0];
                 // stochastic-rounding hash of each stream rating (lane u: its row
                 // and its piece's update key), one vector pass for the stream
@@ -301,17 +313,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                     const uint32_t srb_u = uint32_t(__shfl(int32_t(t_srb), my_t));
                     s_rh = mix32(a.sr_seed ^ mix32(uint32_t(srow) * 0x9E3779B1u + srb_u));
                 }
-                // Each prologue load is preceded by a store that the range check
-                // drops, so the prologue issues memory operations in the same
-                // pattern as the loop and the compiler's wait counts stay as
-                // loose as the prefetch distance allows.
-#pragma unroll
-                for (int d = 0; d < kPf; ++d) {
-                    const int32_t r0 = readlane_i32(srow, min(d, total - 1));
-                    // (distinct offsets: stores to one offset would be merged away)
-                    T::store_b(wrs, kDrop - d * 1024, 0, P{Aa(0), Aa(0)}, 0u, RM);
-                    wp[d] = T::load_raw_b(wrs, woff(r0));
-                }
+
                 int cur_t = 0;
                 int s_cur = readlane_i32(ts, 0);
                 P h, hn;

@@ -117,3 +117,62 @@ extern "C" int nomad_autotune(const nomad_params* params, const int64_t* colptr,
     if (!any) return fail(first_err, "nomad_autotune: every trial failed; first error: " + first_msg);
     return NOMAD_OK;
 }
+
+extern "C" int nomad_autotune_batch(const nomad_batch_problem* probs, int32_t count, int32_t* strata_out,
+                                    double* updates_per_s) {
+    using nomad_host::fail;
+    if (!probs || count <= 0 || count > 256 || !strata_out)
+        return fail(NOMAD_EINVAL, "nomad_autotune_batch: need 1..256 problems and an output array");
+    for (int32_t i = 0; i < count; ++i) {
+        const nomad_params& p = probs[i].params;
+        if (p.mode != NOMAD_MODE_STRAT || p.m <= 0 || p.n <= 0 || p.k <= 0 || !probs[i].colptr)
+            return fail(NOMAD_EINVAL, "nomad_autotune_batch: problems must be valid strat-mode problems");
+    }
+    // start points, drawn once per problem
+    std::vector<std::vector<double>> W0(count), H0(count);
+    for (int32_t i = 0; i < count; ++i) {
+        const nomad_params& p = probs[i].params;
+        std::mt19937_64 rng(p.seed);
+        std::uniform_real_distribution<double> u(0.0, 1.0 / std::sqrt(double(p.k)));
+        W0[i].resize(size_t(p.m) * p.k);
+        H0[i].resize(size_t(p.n) * p.k);
+        for (double& x : W0[i]) x = u(rng);
+        for (double& x : H0[i]) x = u(rng);
+    }
+    double best_rate = -1.0;
+    int32_t best_B = 0;
+    std::string last_msg;
+    int last_rc = NOMAD_OK;
+    for (int32_t B : {64, 128, 256, 512, 1024}) {
+        std::vector<nomad_ctx*> cs(static_cast<size_t>(count), nullptr);
+        int rc = NOMAD_OK;
+        for (int32_t i = 0; i < count && rc == NOMAD_OK; ++i) {
+            nomad_params p = probs[i].params;
+            p.nworkers = std::min(B, p.m);
+            rc = nomad_create(&p, &cs[i]);
+            if (rc == NOMAD_OK) rc = nomad_upload_train(cs[i], probs[i].colptr, probs[i].rowidx, probs[i].val,
+                                                        probs[i].nnz);
+            if (rc == NOMAD_OK) rc = nomad_set_factors(cs[i], W0[i].data(), H0[i].data());
+        }
+        std::vector<nomad_train_stats> st(static_cast<size_t>(count));
+        if (rc == NOMAD_OK) rc = nomad_train_batch(cs.data(), count, 1, 0, st.data());
+        if (rc == NOMAD_OK) rc = nomad_train_batch(cs.data(), count, 1, 0, st.data());
+        if (rc == NOMAD_OK) {
+            double upd = 0.0;
+            for (const auto& x : st) upd += double(x.updates);
+            const double rate = st[0].train_seconds > 0 ? upd / st[0].train_seconds : 0.0;
+            if (rate > best_rate) {
+                best_rate = rate;
+                best_B = B;
+            }
+        } else {
+            last_rc = rc;
+            last_msg = nomad_last_error();
+        }
+        for (nomad_ctx* c : cs) nomad_destroy(c);
+    }
+    if (best_B == 0) return fail(last_rc, "nomad_autotune_batch: every trial failed; last error: " + last_msg);
+    for (int32_t i = 0; i < count; ++i) strata_out[i] = std::min(best_B, probs[i].params.m);
+    if (updates_per_s) *updates_per_s = best_rate;
+    return NOMAD_OK;
+}

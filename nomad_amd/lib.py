@@ -76,6 +76,11 @@ class TuneTrial(C.Structure):
                 ("updates_per_s", C.c_double)]
 
 
+class BatchProblem(C.Structure):
+    _fields_ = [("params", Params), ("colptr", C.POINTER(C.c_int64)), ("rowidx", C.POINTER(C.c_int32)),
+                ("val", C.POINTER(C.c_double)), ("nnz", C.c_int64)]
+
+
 class SynthSpec(C.Structure):
     _fields_ = [
         ("m", C.c_int32), ("n", C.c_int32), ("nnz_train", C.c_int64), ("nnz_test", C.c_int64),
@@ -108,6 +113,7 @@ HIP_SYMBOLS = {
     "nomad_train_batch": (C.c_int, [C.POINTER(_P), C.c_int32, C.c_int64, C.c_int32, C.POINTER(TrainStats)]),
     "nomad_autotune": (C.c_int, [C.POINTER(Params), _i64p, _i32p, _f64p, C.c_int64, C.c_double, C.POINTER(TuneTrial),
                                  C.POINTER(TuneTrial), C.c_int32, _i32p]),
+    "nomad_autotune_batch": (C.c_int, [C.POINTER(BatchProblem), C.c_int32, _i32p, C.POINTER(C.c_double)]),
     "nomad_obj_create": (C.c_int, [C.POINTER(ObjParams), _i64p, _i32p, _f64p, C.c_int64, C.POINTER(_P)]),
     "nomad_obj_eval": (C.c_int, [_P, _P, _P, C.c_double, _P, _P, C.POINTER(C.c_double), _P]),
     "nomad_obj_destroy": (C.c_int, [_P]),
@@ -486,3 +492,32 @@ def autotune(train: Csc, k: int, *, mode: str = "async", storage: str = "fp32", 
     if rc != 0:
         raise NomadError(rc, lib.nomad_last_error().decode())
     return best, list(trials[: min(nt.value, 32)])
+
+
+def autotune_batch(problems, *, storage: str = "bf16", rounding: str = "stochastic", lrate: float = 0.005,
+                   drate: float = 0.05, seed: int = 12345, device: int = 0):
+    """nomad_autotune_batch for [(train Csc, k, reg), ...]: returns (strata per
+    problem, the batch's updates/s with them)."""
+    lib = hip()
+    arr = (BatchProblem * len(problems))()
+    keep = []
+    for i, (tr, k, reg) in enumerate(problems):
+        p = arr[i].params
+        p.m, p.n, p.k = tr.m, tr.n, k
+        p.storage = STORAGE_NAMES[storage]
+        p.mode = MODE_STRAT
+        p.rounding = {"nearest": ROUND_NEAREST, "stochastic": ROUND_STOCHASTIC}[rounding]
+        p.device = device
+        p.lrate, p.drate, p.reg, p.seed = lrate, drate, reg, seed + i
+        cp = np.ascontiguousarray(tr.colptr, np.int64)
+        ri = np.ascontiguousarray(tr.rowidx, np.int32)
+        va = np.ascontiguousarray(tr.val, np.float64)
+        keep += [cp, ri, va]
+        arr[i].colptr, arr[i].rowidx, arr[i].val, arr[i].nnz = (_ptr(cp, C.c_int64), _ptr(ri, C.c_int32),
+                                                                _ptr(va, C.c_double), tr.nnz)
+    out = np.zeros(len(problems), np.int32)
+    rate = C.c_double()
+    rc = lib.nomad_autotune_batch(arr, len(problems), _ptr(out, C.c_int32), C.byref(rate))
+    if rc != 0:
+        raise NomadError(rc, lib.nomad_last_error().decode())
+    return out.tolist(), rate.value

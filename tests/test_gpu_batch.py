@@ -151,3 +151,17 @@ def test_autotune_picks_a_valid_fastest_config(gpu, small, mode):
     ev = c.eval()
     assert np.isfinite(ev.test_sse)
     c.close()
+
+
+def test_autotune_batch_picks_a_valid_common_strata_count(gpu):
+    probs = _problems(gpu, 4, 0.01, first=20)
+    strata, rate = gpu.autotune_batch([(tr, k, reg) for tr, te, k, reg in probs])
+    print(f"[autotune batch] strata {strata}, {rate / 1e9:.3f} G updates/s", flush=True)
+    assert rate > 0
+    for (tr, te, k, reg), B in zip(probs, strata):
+        assert B == min(max(strata), tr.m) and max(strata) in (64, 128, 256, 512, 1024)
+    cs = [_ctx(gpu, tr, te, k, reg, B, "bf16", seed=i) for i, ((tr, te, k, reg), B) in enumerate(zip(probs, strata))]
+    st = gpu.train_batch(cs, 1)
+    assert all(s.updates == p[0].nnz for s, p in zip(st, probs))
+    for c in cs:
+        c.close()
