@@ -21,7 +21,9 @@ separate context that is then discarded.
 
 Prints one JSON line.
 
-Other workloads (not the default line): `--workload ccd` (CCD++ rank-one
+Other workloads (not the default line): `--workload yahoo` (BF16 vs FP32 vs
+FP64 test RMSE after equal epochs at the Yahoo! Music shape, with the
+oracle's FP64 async run on the host), `--workload ccd` (CCD++ rank-one
 sweeps at the Netflix shape), `--workload objective` (objective and gradient
 at 100,000 x 20,000 with 10 M cells, k = 64), and `--workload sweep`: the batched
 hyper-parameter sweep -- 32 independent problems of uneven sizes (users ~
@@ -300,6 +302,59 @@ def objective_workload(a, lib) -> dict:
             "paths": out}
 
 
+def yahoo_workload(a, lib) -> dict:
+    """Yahoo! Music shape (1,000,990 x 624,961, 252.8 M training ratings,
+    integer ratings 0-100, k = 100, reg 1.0): the async kernel with BF16
+    (stochastic rounding), FP32 and FP64 storage for a.epochs epochs each,
+    beside the oracle's FP64 async restatement on the host for the same
+    epochs; test RMSE of each and its gap to the FP64 oracle."""
+    from oracle import oracle as O
+
+    k = 100
+    hyp = dict(lrate=0.001, drate=0.05, reg=1.0)
+    t0 = time.perf_counter()
+    tr, te = lib.synth(lib.preset(3, a.scale))
+    log(f"yahoo: generated {tr.m} x {tr.n}, train {tr.nnz}, test {te.nnz} in {time.perf_counter() - t0:.1f} s")
+    W0, H0 = lib.init_factors(tr.m, tr.n, k, 4, 12345)
+    paths = {}
+    for name, storage, rnd in (("bf16", "bf16", "stochastic"), ("fp32", "fp32", "nearest"),
+                               ("fp64", "fp64", "nearest")):
+        ctx = lib.Context(tr.m, tr.n, k, storage=storage, rounding=rnd, mode="async", nworkers=a.nworkers,
+                          seed=12345, balance=True, **hyp)
+        ctx.upload(tr, te)
+        ctx.set_factors(W0, H0)
+        st = ctx.train(epochs=a.epochs)
+        ev = ctx.eval()
+        rmse = float(np.sqrt(ev.test_sse / ev.test_count))
+        paths[name] = {"test_rmse": rmse, "updates_per_s": st.updates / st.train_seconds,
+                       "train_seconds": st.train_seconds, "epochs": int(st.epochs_done)}
+        log(f"yahoo: gpu {name}: {a.epochs} epochs in {st.train_seconds:.2f} s "
+            f"({st.updates / st.train_seconds / 1e9:.3f} G updates/s), test RMSE {rmse:.5f}")
+        ctx.close()
+    # cpu_baseline leg: the oracle's FP64 async restatement on the host, the
+    # same epochs; its test RMSE is the FP64 reference of the gaps
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    Wc, Hc, st = O.sgd_async(tr, W0, H0, storage=O.FP64, T=threads, seed=12345, epochs=a.epochs, **hyp)
+    cpu_s = time.perf_counter() - t0
+    ro = float(np.sqrt(O.sse(te, Wc, Hc, storage=O.FP64) / te.nnz))
+    log(f"yahoo: cpu fp64 oracle ({threads} threads): {a.epochs} epochs in {cpu_s:.1f} s, test RMSE {ro:.5f}")
+    for v in paths.values():
+        v["gap_vs_fp64_oracle"] = (v["test_rmse"] - ro) / ro
+    head = paths["bf16"]
+    return {"metric": "BF16 vs FP32 vs FP64 test RMSE after equal epochs, Yahoo-shape k=100, 1 GPU",
+            "value": head["updates_per_s"], "unit": "updates/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "bf16 storage, f32 arithmetic", "data": "synthetic", "epochs": a.epochs,
+            "config": {"workload": "yahoo-shape synthetic, async NOMAD token kernel, epoch mode",
+                       "m": tr.m, "n": tr.n, "nnz_train": tr.nnz, "nnz_test": te.nnz, "k": k, **hyp,
+                       "workers": a.nworkers},
+            "paths": paths,
+            "cpu_baseline": {"value": st.updates / cpu_s, "unit": "updates/s", "cores": threads, "kind": "port",
+                             "sample": f"oracle async restatement (orc_sgd_async), FP64 storage and arithmetic, "
+                                       f"{a.epochs} epochs of the full training set",
+                             "test_rmse": ro}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,7 +366,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp32", "bf16"], default=None)
-    ap.add_argument("--workload", choices=["netflix", "sweep", "ccd", "objective"], default="netflix")
+    ap.add_argument("--workload", choices=["netflix", "sweep", "ccd", "objective", "yahoo"], default="netflix")
+    ap.add_argument("--epochs", type=int, default=3, help="yahoo: epochs per path")
     ap.add_argument("--ccd-rank", type=int, default=100)
     ap.add_argument("--ccd-inner", type=int, default=3)
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
@@ -332,7 +388,8 @@ def main() -> int:
     if a.copy_calibrate > 0:
         copy_calibrate(a.copy_calibrate)
     if a.workload != "netflix":
-        fn = {"sweep": sweep, "ccd": ccd_workload, "objective": objective_workload}[a.workload]
+        fn = {"sweep": sweep, "ccd": ccd_workload, "objective": objective_workload,
+              "yahoo": yahoo_workload}[a.workload]
         print(json.dumps(fn(a, lib)), flush=True)
         return 0
     k = 100

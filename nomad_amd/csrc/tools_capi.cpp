@@ -176,9 +176,9 @@ int nomad_tools_preset(int32_t config, double scale, nomad_synth_spec* o) {
             o->nnz_test = scaled(4003960, scale);
             o->rank = 64;
             o->factor_var = 1.0 / std::sqrt(64.0);
-            o->noise_std = 12.0;
             o->center = 50.0;
             o->scale = 20.0;
+            o->noise_std = 12.0 / o->scale;  // N(0, 12) in rating units (noise is added before scaling)
             o->lo = 0.0;
             o->hi = 100.0;
             o->seed = 20250002;
