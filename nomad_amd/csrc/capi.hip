@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <new>
 #include <numeric>
 #include <random>
@@ -407,30 +408,70 @@ void build_partition(nomad_ctx* c) {
 // (Code/nomad/pool.cpp:32-38) with P workers in place of the reference's
 // threads.  Workers with no rating of the column are skipped: their visit in
 // the reference is an empty loop.
+// Re-permutation after a segment (the reference re-shuffles every column's
+// route after each evaluation, nb_main.cpp:313-321): each column's runs are
+// put in a fresh uniformly random order by a Fisher-Yates shuffle.  The runs
+// (the column's non-empty workgroup ranges) do not change, so this is
+// O(visits); columns are shuffled in 64 fixed chunks on up to 16 host
+// threads, each column from its own splitmix64 stream keyed by one draw of
+// the re-permutation generator and the column, so the result does not
+// depend on the thread count.
+uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+void reperm_routes(nomad_ctx* c) {
+    const int32_t n = c->p.n;
+    const uint64_t key = c->rng_reperm();
+    constexpr int kChunks = 64;
+    auto chunk = [&](int q) {
+        const int32_t j0 = int32_t(int64_t(n) * q / kChunks), j1 = int32_t(int64_t(n) * (q + 1) / kChunks);
+        for (int32_t j = j0; j < j1; ++j) {
+            uint64_t st = key ^ (uint64_t(j) * 0xD1B54A32D192ED03ull);
+            VisitDesc* v = c->h_visits.data() + c->h_vptr[j];
+            const int64_t len = c->h_vptr[j + 1] - c->h_vptr[j];
+            for (int64_t i = len - 1; i > 0; --i) {
+                const int64_t r = int64_t(splitmix64(st) % uint64_t(i + 1));
+                std::swap(v[i], v[r]);
+            }
+        }
+    };
+    const int nth = int(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    std::vector<std::thread> th;
+    for (int t = 1; t < nth; ++t)
+        th.emplace_back([&, t] {
+            for (int q = t; q < kChunks; q += nth) chunk(q);
+        });
+    for (int q = 0; q < kChunks; q += nth) chunk(q);
+    for (auto& x : th) x.join();
+}
+
+// Initial routes: per column a shuffle of the workgroups with the route
+// generator (the reference's set_perm, pool.cpp:32-38, with mt19937_64
+// seeded seed + 137, nb_main.cpp:79), restricted to the workgroups that hold
+// ratings of the column; each such (column, workgroup) pair is one run of
+// the CSC column (rows ascending, so a workgroup's rows are contiguous).
 void build_routes(nomad_ctx* c, std::mt19937_64& rng) {
     const int32_t n = c->p.n, P = c->units;
     std::vector<int32_t> perm(P), rank(P);
     std::vector<VisitDesc> runs;
-    const bool first = c->h_vptr.empty();
     std::vector<int64_t> vptr(size_t(n) + 1, 0);
     std::vector<VisitDesc> visits;
-    visits.reserve(c->h_visits.size());
     for (int32_t j = 0; j < n; ++j) {
         for (int32_t q = 0; q < P; ++q) perm[q] = q;
         std::shuffle(perm.begin(), perm.end(), rng);
         for (int32_t q = 0; q < P; ++q) rank[perm[q]] = q;
         runs.clear();
-        if (first) {
-            const int64_t c0 = c->h_colptr[j], c1 = c->h_colptr[j + 1];
-            for (int64_t e = c0; e < c1;) {
-                const int32_t wk = c->row_unit[c->h_rowidx[e]];
-                int64_t f = e;
-                while (f < c1 && c->row_unit[c->h_rowidx[e]] == c->row_unit[c->h_rowidx[f]]) ++f;
-                runs.push_back(VisitDesc{e, int32_t(f - e), wk});
-                e = f;
-            }
-        } else {
-            for (int64_t v = c->h_vptr[j]; v < c->h_vptr[j + 1]; ++v) runs.push_back(c->h_visits[v]);
+        const int64_t c0 = c->h_colptr[j], c1 = c->h_colptr[j + 1];
+        for (int64_t e = c0; e < c1;) {
+            const int32_t wk = c->row_unit[c->h_rowidx[e]];
+            int64_t f = e;
+            while (f < c1 && c->row_unit[c->h_rowidx[f]] == wk) ++f;
+            runs.push_back(VisitDesc{e, int32_t(f - e), wk});
+            e = f;
         }
         std::sort(runs.begin(), runs.end(),
                   [&](const VisitDesc& a, const VisitDesc& b) { return rank[a.worker] < rank[b.worker]; });
@@ -503,7 +544,7 @@ int setup_async(nomad_ctx* c) {
         c->cycle_base = 0;
     } else {
         NOMAD_HIP_TRY(hipMemcpy(tok.data(), c->d_tok, size_t(n) * sizeof(TokInfo), hipMemcpyDeviceToHost));
-        build_routes(c, c->rng_reperm);
+        reperm_routes(c);
         // drop whatever is still queued: head = tail on every ring
         NOMAD_HIP_TRY(hipMemcpy(c->d_qhead, c->d_qtail, size_t(P) * nomad_k::kQStride * 4, hipMemcpyDeviceToDevice));
     }
