@@ -238,17 +238,18 @@ def ccd_workload(a, lib) -> dict:
     W0, _ = lib.init_factors(tr.m, tr.n, k, 1, 12345)
     H0 = np.zeros((tr.n, k))
     _, _, obj, rmse, st = lib.ccd(tr, W0, H0, 0.05, 1, a.ccd_inner, eps=1e-3, test=te)
-    log(f"ccd: {st.ranks_done} ranks, {st.inner_done} inner iterations in {st.seconds:.3f} s, "
-        f"final obj {obj[-1]:.6g} test RMSE {rmse[-1]:.6f}")
-    gbs = st.alg_bytes / st.seconds / 1e9
-    return {"metric": "CCD++ rank-one sweep, Netflix-shape, FP64, 1 GPU", "value": st.inner_done / st.seconds,
+    log(f"ccd: {st.ranks_done} ranks, {st.inner_done} inner iterations in {st.solve_seconds:.3f} s "
+        f"({st.seconds:.3f} s with host set-up), final obj {obj[-1]:.6g} test RMSE {rmse[-1]:.6f}")
+    gbs = st.alg_bytes / st.solve_seconds / 1e9
+    return {"metric": "CCD++ rank-one sweep, Netflix-shape, FP64, 1 GPU", "value": st.inner_done / st.solve_seconds,
             "unit": "inner iterations/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
-            "data": "synthetic", "seconds": st.seconds,
+            "data": "synthetic", "seconds": st.solve_seconds, "seconds_with_setup": st.seconds,
             "config": {"workload": f"ccd++ k={k}, 1 outer iteration, {a.ccd_inner} inner, lambda 0.05",
                        "m": tr.m, "n": tr.n, "nnz_train": tr.nnz},
             "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": gbs / HBM_PEAK_GBS, "traffic": None,
-                         "note": "wall time of the call (host set-up included)"},
+                         "note": "wall time of the rank / inner-iteration loop (host early-stop reads "
+                                 "included, host set-up excluded)"},
             "final_test_rmse": float(rmse[-1])}
 
 

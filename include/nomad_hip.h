@@ -238,12 +238,15 @@ typedef struct nomad_ccd_params {
 } nomad_ccd_params;
 
 typedef struct nomad_ccd_stats {
-    int64_t ranks_done;  /* ranks updated (not skipped)             */
-    int64_t inner_done;  /* inner iterations run                    */
-    double seconds;      /* wall time of the call                   */
-    double alg_bytes;    /* algorithmic bytes: per inner iteration
-                            2 nnz (4 + 8) + 16 (m + n); per rank
-                            32 (nnz + test nnz) (residual updates)  */
+    int64_t ranks_done;    /* ranks updated (not skipped)             */
+    int64_t inner_done;    /* inner iterations run                    */
+    double seconds;        /* wall time of the call                   */
+    double alg_bytes;      /* algorithmic bytes: per inner iteration
+                              2 nnz (4 + 8) + 16 (m + n); per rank
+                              32 (nnz + test nnz) (residual updates)  */
+    double solve_seconds;  /* wall time of the rank / inner-iteration
+                              loop alone (after the host builds the
+                              CSR, uploads and the residual init)     */
 } nomad_ccd_stats;
 
 int nomad_ccd(const nomad_ccd_params* params, const int64_t* colptr, const int32_t* rowidx, const double* val,

@@ -19,7 +19,8 @@
 //     runs their SGD steps as one stream (nb_main.cpp:520-542; rows ascending
 //     within a column, columns in inbox order).  Rows of W are prefetched 8-32
 //     ratings ahead across column boundaries; a row that occurs twice within
-//     that distance takes the value of its earlier update from registers.
+//     that distance takes the value of its earlier update from a per-wave LDS
+//     ring of the rows the stream has stored.
 //     Slots whose piece is not inside the window are run one at a time,
 //     reading their ratings from HBM and refilling the window;
 //   * h goes back to the slot and the slot to the wave owning the column's

@@ -346,6 +346,8 @@ extern "C" int nomad_ccd(const nomad_ccd_params* p, const int64_t* colptr, const
     if (rmse)
         for (int64_t q = 0; q < int64_t(p->iters) * k; ++q) rmse[q] = std::nan("");
     int64_t ranks_done = 0, inner_done = 0;
+    if ((rc = fetch())) return rc;  // set-up work finished
+    const auto t_solve = std::chrono::steady_clock::now();
     for (int oiter = 1; oiter <= p->iters; ++oiter) {
         double fundec_max = 0;  // both restart with every outer iteration
         int early_stop = 0;
@@ -385,6 +387,7 @@ extern "C" int nomad_ccd(const nomad_ccd_params* p, const int64_t* colptr, const
             ++ranks_done;
         }
     }
+    const double solve_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_solve).count();
     if (hipMemcpy(Wt.data(), d_Wt, Wt.size() * 8, hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(Ht.data(), d_Ht, Ht.size() * 8, hipMemcpyDeviceToHost) != hipSuccess)
         return fail(NOMAD_EHIP, "nomad_ccd: copy of the factors failed");
@@ -396,6 +399,7 @@ extern "C" int nomad_ccd(const nomad_ccd_params* p, const int64_t* colptr, const
         st->ranks_done = ranks_done;
         st->inner_done = inner_done;
         st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+        st->solve_seconds = solve_s;
         // per inner iteration: both passes read index + residual of every
         // rating and write both vectors; per rank: residual read + write for
         // the add and the subtract, training and test

@@ -205,8 +205,6 @@ struct Store<NOMAD_STORE_FP32> {
         const u32x2_t v = {__float_as_uint(p.x), __float_as_uint(p.y)};
         __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, 0);
     }
-    // the value a later load of a row stored with store() returns
-    static __device__ __forceinline__ pair rounded(pair p, uint32_t, int) { return p; }
     // device-coherent (sc1) variants for rows that move between workers
     static __device__ __forceinline__ pair load_dc(const elem* row, int lane, int npairs) {
         const uint64_t* src = lane < npairs ? reinterpret_cast<const uint64_t*>(row)
@@ -243,7 +241,6 @@ struct Store<NOMAD_STORE_FP64> {
         const u32x4_t v = {uint32_t(a), uint32_t(a >> 32), uint32_t(b), uint32_t(b >> 32)};
         __builtin_amdgcn_raw_buffer_store_b128(v, row_rsrc(row, npairs * 16), lane * 16, 0, CP);
     }
-    static __device__ __forceinline__ pair rounded(pair p, uint32_t, int) { return p; }
     static constexpr int kPairBytes = 16;
     static __device__ __forceinline__ pair load_b(__amdgpu_buffer_rsrc_t r, int voff, int soff = 0) {
         const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
@@ -355,17 +352,6 @@ struct Store<NOMAD_STORE_BF16> {
             hi = f32_to_bf16_rne(p.y);
         }
         __builtin_amdgcn_raw_buffer_store_b32(lo | (hi << 16), r, voff, soff, 0);
-    }
-    static __device__ __forceinline__ pair rounded(pair p, uint32_t rnd, int round_mode) {
-        uint32_t lo, hi;
-        if (round_mode == NOMAD_ROUND_STOCHASTIC) {
-            lo = f32_to_bf16_sr(p.x, rnd);
-            hi = f32_to_bf16_sr(p.y, rnd >> 16);
-        } else {
-            lo = f32_to_bf16_rne(p.x);
-            hi = f32_to_bf16_rne(p.y);
-        }
-        return pair{__uint_as_float(lo << 16), __uint_as_float(hi << 16)};
     }
     static __device__ __forceinline__ pair load_dc(const elem* row, int lane, int npairs) {
         const uint32_t* src = lane < npairs ? reinterpret_cast<const uint32_t*>(row)

@@ -64,6 +64,7 @@ class CcdParams(C.Structure):
 class CcdStats(C.Structure):
     _fields_ = [
         ("ranks_done", C.c_int64), ("inner_done", C.c_int64), ("seconds", C.c_double), ("alg_bytes", C.c_double),
+        ("solve_seconds", C.c_double),
     ]
 
 
