@@ -51,7 +51,6 @@ template <int S, int NW>
 constexpr int stream_depth() {
     return (64 * 1024) / (NW * kWave * 2 * int(sizeof(typename Store<S>::acc)));
 }
-constexpr int kGroup = 8;  // ratings per unrolled group (stream depths are multiples)
 
 // W rows in flight inside one long piece: deep enough to cover HBM latency
 // with one wave per SIMD (4 waves per workgroup), less where more waves share
@@ -158,8 +157,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
     static_assert(SLOTS % 32 == 0 && (SLOTS & (SLOTS - 1)) == 0, "SLOTS must be a power of two >= 32");
     static_assert(NW <= 16 && kBatch <= kWave, "layout limits");
     constexpr int kPf = stream_depth<S, NW>();
-    constexpr int kG = kPf < kGroup ? kPf : kGroup;
-    static_assert(kPf % kG == 0, "stream depth must be a multiple of the group");
+    static_assert(kWave % kPf == 0 && kWave % long_prefetch_depth<NW>() == 0, "depths must divide the wave");
     __shared__ AsyncShared<S, NW, SLOTS> sh;
     const int g = blockIdx.x;
     // wave-uniform values are made scalar so that branches on them stay
@@ -170,6 +168,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
     const int npairs = a.ldf >> 1;
     const int64_t ldf = a.ldf;
     const uint32_t lkey = sr_lane_key(a.sr_seed, uint32_t(lane));
+    const uint32_t lkey_lo = lkey & 0xFFFFu, lkey_hi = lkey >> 16;
     const int32_t* wb_g = a.wbound + int64_t(g) * NW;
     const int32_t row_lo = __builtin_amdgcn_readfirstlane(wb_g[wave]);
     const int32_t row_hi = __builtin_amdgcn_readfirstlane(wb_g[wave + 1]);
@@ -182,18 +181,18 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
     const int32_t* __restrict__ rowidx = a.rowidx;
     const R* __restrict__ val = static_cast<const R*>(a.val);
     const Aa reg = Aa(a.reg);
-    // This wave's rows of W through one buffer resource.  The hardware range
-    // check covers the per-lane offset (not the scalar one), so the row's byte
-    // offset is added to it: lanes past the row, any row outside the block and
-    // the kDrop offset fall outside the range (loads read zero, stores are
-    // dropped).
+    // This wave's rows of W through one buffer resource.  A row's byte offset
+    // within the block is the scalar offset of its load or store, always a
+    // row of this wave; the hardware range check covers the per-lane offset
+    // (not the scalar one), which is the lane's pair offset, 2^31 for lanes
+    // past the row, or kDrop with a scalar offset of 0 for a store to drop
+    // (out-of-range loads read zero, stores are dropped).
     const int rowbytes = a.ldf * int(sizeof(E));
     const __amdgpu_buffer_rsrc_t wrs = __builtin_amdgcn_make_buffer_rsrc(
         W + int64_t(row_lo) * ldf, 0, (row_hi - row_lo) * rowbytes, 0x00020000);
-    // (unsigned arithmetic: offsets past 2^31 wrap by design)
     const uint32_t wvoff = lane < npairs ? uint32_t(lane * T::kPairBytes) : 0x80000000u;
-    auto woff = [&](int32_t r) { return int(wvoff + uint32_t(r - row_lo) * uint32_t(rowbytes)); };
     const int kDrop = int(wvoff + 0x7FFFFFFFu);  // every lane out of range (also kDrop - d * 1024, d < 64)
+    const Aa sel63 = lane == kWave - 1 ? Aa(1) : Aa(0);  // where a rating enters the dot product's tree
 
     for (int i = threadIdx.x; i < NW * SLOTS; i += blockDim.x) (&sh.inbox[0][0])[i] = 0u;
     if (threadIdx.x < NW) sh.itail[threadIdx.x] = 0u;
@@ -273,6 +272,10 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                     srow = sh.wrow[ms][mo];
                     sval = sh.wval[ms][mo];
                 }
+                // byte offset of each rating's row inside this wave's block: it
+                // goes to the scalar offset of the row's load and store (always
+                // a row of this wave; the per-lane offset keeps the range check)
+                const int32_t soffv = lane < total ? (srow - row_lo) * rowbytes : 0;
                 // The stream's first W rows are requested as soon as the rows are
                 // known; the rest of the set-up runs while they arrive.  Each
                 // prologue load is preceded by a store that the range check
@@ -281,12 +284,13 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                 // loose as the prefetch distance allows.
                 const uint64_t t_c0 = clk();
                 typename T::raw wp[kPf];
+                int32_t soff[kPf];  // row offset of the load into wp[d] (scalar)
 #pragma unroll
                 for (int d = 0; d < kPf; ++d) {
-                    const int32_t r0 = readlane_i32(srow, min(d, total - 1));
+                    soff[d] = readlane_i32(soffv, min(d, total - 1));
                     // (distinct offsets: stores to one offset would be merged away)
                     T::store_b(wrs, kDrop - d * 1024, 0, P{Aa(0), Aa(0)}, 0u, RM);
-                    wp[d] = T::load_raw_b(wrs, woff(r0));
+                    wp[d] = T::load_raw_b(wrs, int(wvoff), soff[d]);
                 }
                 Aa t_step = Aa(0);
                 uint32_t t_srb = 0;
@@ -306,7 +310,9 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                     const int32_t o = __shfl_up(srow, b);
                     if (lane >= b && lane < total && o == srow) pdist = b;
                 }
-                typename T::raw* const vring = &sh.vic[wave][0][0];
+                // scalar mask over the stream's ratings: the first rating of each
+                // piece after the first
+                const uint64_t pmask = __ballot(lane > 0 && lane < total && __shfl_up(my_t, 1) != my_t);
                 // stochastic-rounding hash of each stream rating (lane u: its row
                 // and its piece's update key), one vector pass for the stream
                 uint32_t s_rh = 0;
@@ -317,70 +323,53 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
 
                 int cur_t = 0;
                 int s_cur = readlane_i32(ts, 0);
-                P h, hn;
+                P h;
                 h.x = sh.h[s_cur][2 * lane];
                 h.y = sh.h[s_cur][2 * lane + 1];
-                hn = h;
-                if (nt > 1) {
-                    const int s1 = readlane_i32(ts, 1);
-                    hn.x = sh.h[s1][2 * lane];
-                    hn.y = sh.h[s1][2 * lane + 1];
-                }
                 // per piece: c1 = 1 - step * reg and -step (the step's two coefficients)
                 const Aa t_c1 = Aa(1) - t_step * reg;
                 Aa c1 = readlane_val(t_c1, 0);
-                Aa nstep = -readlane_val(t_step, 0);
-                // kPf ratings per iteration in groups of kGroup.  A group that
-                // starts past the stream leaves both loops at once (an exit
-                // edge, so the loop keeps exact wait counts); slots of the last
-                // group past the stream are idle ratings whose store falls
-                // outside its buffer range and whose h update is discarded.
+                Aa nstep = -__shfl(t_step, 0);  // (kept in a vector register: the error is scalar)
+                // lane u: the scalar offset of the row that rating u prefetches
+                // for rating u + kPf (the last rating's row past the stream)
+                const int32_t soffN = __shfl(soffv, min(lane + kPf, total - 1));
+                // kPf ratings per iteration.  The loop is left before the first
+                // rating past the stream (an exit edge: every path through the
+                // body issues the same memory operations, so the compiler's
+                // wait counts stay exact).
                 for (int u0 = 0;; u0 += kPf) {
+                    const int left = total - u0;
+                    const uint32_t pm = uint32_t(pmask >> u0);  // piece starts of this iteration
 #pragma unroll
-                    for (int gq = 0; gq < kPf / kG; ++gq) {
-                        if (u0 + gq * kG >= total) goto stream_done;
-#pragma unroll
-                        for (int q = 0; q < kG; ++q) {
-                            const int d = gq * kG + q;
-                            const int u = u0 + d;
-                            const bool real = u < total;
-                            const int uc = real ? u : total - 1;
-                            const int32_t i = readlane_i32(srow, uc);
-                            const int t = readlane_i32(my_t, uc);
-                            if (t != cur_t) {  // next piece: park h, take the next column's
-                                sh.h[s_cur][2 * lane] = h.x;
-                                sh.h[s_cur][2 * lane + 1] = h.y;
-                                h = hn;
-                                cur_t = t;
-                                s_cur = readlane_i32(ts, t);
-                                c1 = readlane_val(t_c1, t);
-                                nstep = -readlane_val(t_step, t);
-                                if (t + 1 < nt) {
-                                    const int sn = readlane_i32(ts, t + 1);
-                                    hn.x = sh.h[sn][2 * lane];
-                                    hn.y = sh.h[sn][2 * lane + 1];
-                                }
-                            }
-                            // a row updated pd ratings ago is read back from the ring;
-                            // it goes to registers of its own, as writing it over
-                            // wp[d] would first wait for wp[d]'s load
-                            const int pd = readlane_i32(pdist, uc);
-                            P fw = P{Aa(0), Aa(0)};
-                            if (pd != 0) fw = T::unpack(vring[((d - pd + kPf) % kPf) * kWave + lane]);
-                            const P wl = T::unpack(wp[d]);
-                            P w;
-                            w.x = pd != 0 ? fw.x : wl.x;
-                            w.y = pd != 0 ? fw.y : wl.y;
-                            // idle slots: c1 = 1, step 0 leave h as it is
-                            sgd_step_c<P, Aa>(w, h, Aa(readlane_val(sval, uc)), real ? c1 : Aa(1), real ? nstep : Aa(0));
-                            // = sr_bits_k(a.sr_seed, i, key of piece t, lkey)
-                            const uint32_t rnd = uint32_t(readlane_i32(int32_t(s_rh), uc)) ^ lkey;
-                            const typename T::raw wr = T::pack(w, rnd, RM);
-                            T::store_raw_b(wrs, real ? woff(i) : kDrop, wr);
-                            vring[d * kWave + lane] = wr;
-                            const int un = min(u + kPf, total - 1);
-                            wp[d] = T::load_raw_b(wrs, woff(readlane_i32(srow, un)));
+                    for (int d = 0; d < kPf; ++d) {
+                        if (d >= left) goto stream_done;
+                        const int u = u0 + d;
+                        if ((pm >> d) & 1u) {  // next piece: park h, take the next column's
+                            sh.h[s_cur][2 * lane] = h.x;
+                            sh.h[s_cur][2 * lane + 1] = h.y;
+                            ++cur_t;
+                            s_cur = readlane_i32(ts, cur_t);
+                            h.x = sh.h[s_cur][2 * lane];
+                            h.y = sh.h[s_cur][2 * lane + 1];
+                            c1 = readlane_val(t_c1, cur_t);
+                            nstep = -__shfl(t_step, cur_t);
                         }
+                        // a row updated pd ratings ago is read back from the ring
+                        // of updated rows
+                        P w = T::unpack(wp[d]);
+                        const int pd = readlane_i32(pdist, u);
+                        P fw = P{Aa(0), Aa(0)};
+                        if (pd != 0) fw = T::unpack(sh.vic[wave][(d - pd + kPf) % kPf][lane]);
+                        w.x = pd != 0 ? fw.x : w.x;
+                        w.y = pd != 0 ? fw.y : w.y;
+                        sgd_step_b<P, Aa>(w, h, Aa(readlane_val(sval, u)), sel63, c1, nstep);
+                        // = sr_bits_k(a.sr_seed, row, key of the piece, lkey)
+                        const typename T::raw wr =
+                            T::template pack_k<RM>(w, uint32_t(readlane_i32(int32_t(s_rh), u)), lkey_lo, lkey_hi);
+                        T::store_raw_b(wrs, int(wvoff), wr, soff[d]);
+                        sh.vic[wave][d][lane] = wr;
+                        soff[d] = readlane_i32(soffN, u);
+                        wp[d] = T::load_raw_b(wrs, int(wvoff), soff[d]);
                     }
                 }
             stream_done:
@@ -484,27 +473,45 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
             constexpr int kPfL = long_prefetch_depth<NW>();
             const int64_t cbase = int64_t(vstart);
             int32_t c0 = cur;  // range offset of chunk A
-            int32_t rA = INT_MAX, rB = INT_MAX;
-            R vA = R(0), vB = R(0);
-            if (c0 + lane < vlen) {
-                rA = rowidx[cbase + c0 + lane];
-                vA = val[cbase + c0 + lane];
-            }
-            if (c0 + kWave + lane < vlen) {
-                rB = rowidx[cbase + c0 + kWave + lane];
-                vB = val[cbase + c0 + kWave + lane];
-            }
+            // the 64 ratings from range offset o (rows past the range: INT_MAX).
+            // The loads are not under a branch: a load only some lanes issue
+            // leaves the compiler unsure of what is in flight, and the loop
+            // after a chunk change would then start with a full wait.
+            auto load_chunk = [&](int32_t o, int32_t& r, R& v) {
+                const int64_t e = cbase + min(o + lane, vlen - 1);
+                const int32_t r0 = rowidx[e];
+                const R v0 = val[e];
+                const bool in = o + lane < vlen;
+                r = in ? r0 : INT_MAX;
+                v = in ? v0 : R(0);
+            };
+            int32_t rA, rB;
+            R vA, vB;
+            load_chunk(c0, rA, vA);
+            load_chunk(c0 + kWave, rB, vB);
             const uint64_t t_c0 = clk();
-            // rows past the piece (another wave's, or the end of the range) are
-            // never updated; their prefetch slots re-read the piece's first row
-            const int32_t r_first = readlane_i32(rA, 0);
+            // Scalar byte offsets of the rows (see the stream): for lane u of
+            // chunk A its own row, and for the prefetch the row kPfL ratings
+            // ahead (in chunk B past A's end).  Rows past the piece (another
+            // wave's, or past the range) are never updated; their prefetch
+            // slots read the wave's first row.
+            auto row_offset = [&](int32_t r) { return r < row_hi ? (r - row_lo) * rowbytes : 0; };
+            auto ahead_offsets = [&]() {
+                const int32_t ra = __shfl(rA, (lane + kPfL) & (kWave - 1));
+                const int32_t rb = __shfl(rB, (lane + kPfL) & (kWave - 1));
+                return row_offset(lane + kPfL < kWave ? ra : rb);
+            };
+            int32_t offN = ahead_offsets();
             typename T::raw wp[kPfL];
+            int32_t soff[kPfL];
+            {
+                const int32_t offA = row_offset(rA);
 #pragma unroll
-            for (int d = 0; d < kPfL; ++d) {
-                int32_t rr = readlane_i32(rA, d);
-                rr = rr < row_hi ? rr : r_first;
-                T::store_b(wrs, kDrop - d * 1024, 0, P{Aa(0), Aa(0)}, 0u, RM);  // dropped: see the stream prologue
-                wp[d] = T::load_raw_b(wrs, woff(rr));
+                for (int d = 0; d < kPfL; ++d) {
+                    soff[d] = readlane_i32(offA, d);
+                    T::store_b(wrs, kDrop - d * 1024, 0, P{Aa(0), Aa(0)}, 0u, RM);  // dropped: see the stream prologue
+                    wp[d] = T::load_raw_b(wrs, int(wvoff), soff[d]);
+                }
             }
             // stochastic-rounding hash of each chunk-A rating (as in the stream)
             auto chunk_hash = [&](int32_t rows) -> uint32_t {
@@ -514,46 +521,49 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                     return 0u;
             };
             uint32_t rhA = chunk_hash(rA);
+            const Aa nstep = -step;
             // piece ratings in chunk A: rows ascend, so they are a prefix
             int32_t cntA = __popcll(__ballot(rA < row_hi));
             int32_t cnt = 0;
+            // full groups of kPfL ratings (u0 + d < kWave); the piece's last
+            // cntA mod kPfL ratings follow the chunk loop
+            auto rating = [&](int uu, int d) {
+                P w = T::unpack(wp[d]);
+                sgd_step_b<P, Aa>(w, h, Aa(readlane_val(vA, uu)), sel63, c1, nstep);
+                // = sr_bits_k(a.sr_seed, row, sr_b, lkey)
+                T::store_raw_b(wrs, int(wvoff),
+                               T::template pack_k<RM>(w, uint32_t(readlane_i32(int32_t(rhA), uu)), lkey_lo, lkey_hi),
+                               soff[d]);
+                soff[d] = readlane_i32(offN, uu);
+                wp[d] = T::load_raw_b(wrs, int(wvoff), soff[d]);
+            };
             for (;;) {
-                // groups of kPfL; idle ratings past the piece as in the stream above
-                for (int u0 = 0; u0 < cntA; u0 += kPfL) {
+                const int full = cntA & ~(kPfL - 1);
+                for (int u0 = 0; u0 < full; u0 += kPfL) {
 #pragma unroll
-                    for (int d = 0; d < kPfL; ++d) {
-                        const int uu = u0 + d;
-                        const bool real = uu < cntA;
-                        const int uc = real ? uu : cntA - 1;
-                        const int32_t i = readlane_i32(rA, uc);
-                        P w = T::unpack(wp[d]);
-                        sgd_step_c<P, Aa>(w, h, Aa(readlane_val(vA, uc)), real ? c1 : Aa(1), real ? -step : Aa(0));
-                        // = sr_bits_k(a.sr_seed, i, sr_b, lkey)
-                        const uint32_t rnd = uint32_t(readlane_i32(int32_t(rhA), uc)) ^ lkey;
-                        T::store_b(wrs, real ? woff(i) : kDrop, 0, w, rnd, RM);
-                        const int tg = uu + kPfL;  // < 2 * kWave
-                        const int32_t ra = readlane_i32(rA, tg & (kWave - 1));
-                        const int32_t rb = readlane_i32(rB, tg & (kWave - 1));
-                        int32_t rn = tg < kWave ? ra : rb;
-                        rn = rn < row_hi ? rn : i;
-                        wp[d] = T::load_raw_b(wrs, woff(rn));
-                    }
+                    for (int d = 0; d < kPfL; ++d) rating(u0 + d, d);
                 }
                 cnt += cntA;
-                if (cntA < kWave) break;
+                if (cntA < kWave) break;  // the piece ends in this chunk
                 // the piece goes on in the next chunk
                 rA = rB;
                 vA = vB;
                 rhA = chunk_hash(rA);
                 c0 += kWave;
-                rB = INT_MAX;
-                vB = R(0);
-                if (c0 + kWave + lane < vlen) {
-                    rB = rowidx[cbase + c0 + kWave + lane];
-                    vB = val[cbase + c0 + kWave + lane];
-                }
+                load_chunk(c0 + kWave, rB, vB);
+                offN = ahead_offsets();
                 cntA = __popcll(__ballot(rA < row_hi));
                 if (cntA == 0) break;
+            }
+            {
+                // the piece's last cntA mod kPfL ratings, after its chunk loop
+                // (here the exits leave the piece, so no loop sees them)
+                const int full = cntA & ~(kPfL - 1);
+#pragma unroll
+                for (int d = 0; d < kPfL; ++d) {
+                    if (full + d >= cntA) break;
+                    rating(full + d, d);
+                }
             }
             // first rating past the piece: position cntA of chunk A
             const int stop_u = cntA;

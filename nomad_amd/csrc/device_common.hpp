@@ -43,14 +43,23 @@ __device__ __forceinline__ double dpp_f64(double x) {
     return __builtin_bit_cast(double, (uint64_t(uint32_t(hi)) << 32) | uint32_t(lo));
 }
 
-__device__ __forceinline__ float wave_tree_sum(float x) {
+// The tree of the two broadcast steps is written with every row enabled:
+// rows 0 and 2 (row_bcast:15) and rows 0 and 1 (row_bcast:31) then also take
+// an addition, but no lane that lane 63's total reads is changed before it is
+// read, so lane 63 holds the same bits as with row masks 0xA and 0xC, and each
+// step is one DPP add instead of a move, a DPP move and an add.
+__device__ __forceinline__ float wave_tree_total(float x) {
     x = x + dpp_f32<0xB1, 0xF>(x);
     x = x + dpp_f32<0x4E, 0xF>(x);
     x = x + dpp_f32<0x114, 0xF>(x);
     x = x + dpp_f32<0x118, 0xF>(x);
-    x = x + dpp_f32<0x142, 0xA>(x);
-    x = x + dpp_f32<0x143, 0xC>(x);
-    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), 63));
+    x = x + dpp_f32<0x142, 0xF>(x);
+    x = x + dpp_f32<0x143, 0xF>(x);
+    return x;  // the total in lane 63
+}
+
+__device__ __forceinline__ float wave_tree_sum(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, wave_tree_total(x)), 63));
 }
 
 __device__ __forceinline__ double wave_tree_sum(double x) {
@@ -188,17 +197,20 @@ struct Store<NOMAD_STORE_FP32> {
     // a loaded pair as it sits in registers before use (see Store<BF16>)
     using raw = pair;
     template <int CP = 0>
-    static __device__ __forceinline__ raw load_raw_b(__amdgpu_buffer_rsrc_t r, int voff) {
-        const u32x2_t v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, CP);
+    static __device__ __forceinline__ raw load_raw_b(__amdgpu_buffer_rsrc_t r, int voff, int soff = 0) {
+        const u32x2_t v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, CP);
         return pair{__uint_as_float(v.x), __uint_as_float(v.y)};
     }
     static __device__ __forceinline__ pair unpack(raw v) { return v; }
     // the stored form of a pair (rounding for bf16) and its store
     static __device__ __forceinline__ raw pack(pair p, uint32_t, int) { return p; }
+    // (the rounding key split as in Store<BF16>::pack_k; unused here)
+    template <int RM>
+    static __device__ __forceinline__ raw pack_k(pair p, uint32_t, uint32_t, uint32_t) { return p; }
     template <int CP = 0>
-    static __device__ __forceinline__ void store_raw_b(__amdgpu_buffer_rsrc_t r, int voff, raw p) {
+    static __device__ __forceinline__ void store_raw_b(__amdgpu_buffer_rsrc_t r, int voff, raw p, int soff = 0) {
         const u32x2_t v = {__float_as_uint(p.x), __float_as_uint(p.y)};
-        __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, 0, CP);
+        __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, CP);
     }
     static __device__ __forceinline__ void store_b(__amdgpu_buffer_rsrc_t r, int voff, int soff, pair p, uint32_t,
                                                    int) {
@@ -249,19 +261,22 @@ struct Store<NOMAD_STORE_FP64> {
     }
     using raw = pair;
     template <int CP = 0>
-    static __device__ __forceinline__ raw load_raw_b(__amdgpu_buffer_rsrc_t r, int voff) {
-        const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, CP);
+    static __device__ __forceinline__ raw load_raw_b(__amdgpu_buffer_rsrc_t r, int voff, int soff = 0) {
+        const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, CP);
         return pair{__builtin_bit_cast(double, (uint64_t(v.y) << 32) | v.x),
                     __builtin_bit_cast(double, (uint64_t(v.w) << 32) | v.z)};
     }
     static __device__ __forceinline__ pair unpack(raw v) { return v; }
     // the stored form of a pair (rounding for bf16) and its store
     static __device__ __forceinline__ raw pack(pair p, uint32_t, int) { return p; }
+    // (the rounding key split as in Store<BF16>::pack_k; unused here)
+    template <int RM>
+    static __device__ __forceinline__ raw pack_k(pair p, uint32_t, uint32_t, uint32_t) { return p; }
     template <int CP = 0>
-    static __device__ __forceinline__ void store_raw_b(__amdgpu_buffer_rsrc_t r, int voff, raw p) {
+    static __device__ __forceinline__ void store_raw_b(__amdgpu_buffer_rsrc_t r, int voff, raw p, int soff = 0) {
         const uint64_t a = __builtin_bit_cast(uint64_t, p.x), b = __builtin_bit_cast(uint64_t, p.y);
         const u32x4_t v = {uint32_t(a), uint32_t(a >> 32), uint32_t(b), uint32_t(b >> 32)};
-        __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, 0, CP);
+        __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, soff, CP);
     }
     static __device__ __forceinline__ void store_b(__amdgpu_buffer_rsrc_t r, int voff, int soff, pair p, uint32_t,
                                                    int) {
@@ -322,24 +337,36 @@ struct Store<NOMAD_STORE_BF16> {
     // prefetch.
     using raw = uint32_t;
     template <int CP = 0>
-    static __device__ __forceinline__ raw load_raw_b(__amdgpu_buffer_rsrc_t r, int voff) {
-        return __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, CP);
+    static __device__ __forceinline__ raw load_raw_b(__amdgpu_buffer_rsrc_t r, int voff, int soff = 0) {
+        return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, CP);
     }
     static __device__ __forceinline__ pair unpack(raw u) { return pair{bf16_lo(u), bf16_hi(u)}; }
+    // (stochastic: f32_to_bf16_sr of both elements, the two upper halves
+    // joined by one byte permute)
     static __device__ __forceinline__ raw pack(pair p, uint32_t rnd, int round_mode) {
-        uint32_t lo, hi;
         if (round_mode == NOMAD_ROUND_STOCHASTIC) {
-            lo = f32_to_bf16_sr(p.x, rnd);
-            hi = f32_to_bf16_sr(p.y, rnd >> 16);
-        } else {
-            lo = f32_to_bf16_rne(p.x);
-            hi = f32_to_bf16_rne(p.y);
+            const uint32_t lo = __float_as_uint(p.x) + (rnd & 0xFFFFu);
+            const uint32_t hi = __float_as_uint(p.y) + (rnd >> 16);
+            return __builtin_amdgcn_perm(hi, lo, 0x07060302u);
         }
-        return lo | (hi << 16);
+        return f32_to_bf16_rne(p.x) | (f32_to_bf16_rne(p.y) << 16);
+    }
+    // pack(p, s ^ key, round_mode) with the wave-uniform hash s and the lane
+    // key given as halves klo = key & 0xFFFF, khi = key >> 16: each element's
+    // rounding bits are (s's half ^ key's half), added in one xor-add.
+    template <int RM>
+    static __device__ __forceinline__ raw pack_k(pair p, uint32_t s, uint32_t klo, uint32_t khi) {
+        if constexpr (RM == NOMAD_ROUND_STOCHASTIC) {
+            const uint32_t lo = __float_as_uint(p.x) + ((s & 0xFFFFu) ^ klo);
+            const uint32_t hi = __float_as_uint(p.y) + ((s >> 16) ^ khi);
+            return __builtin_amdgcn_perm(hi, lo, 0x07060302u);
+        } else {
+            return f32_to_bf16_rne(p.x) | (f32_to_bf16_rne(p.y) << 16);
+        }
     }
     template <int CP = 0>
-    static __device__ __forceinline__ void store_raw_b(__amdgpu_buffer_rsrc_t r, int voff, raw v) {
-        __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, 0, CP);
+    static __device__ __forceinline__ void store_raw_b(__amdgpu_buffer_rsrc_t r, int voff, raw v, int soff = 0) {
+        __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, soff, CP);
     }
     static __device__ __forceinline__ void store_b(__amdgpu_buffer_rsrc_t r, int voff, int soff, pair p,
                                                    uint32_t rnd, int round_mode) {
@@ -396,6 +423,22 @@ __device__ __forceinline__ void sgd_step_c(P& w, P& h, A r, A c1, A nstep) {
     h.y = fma_t(c2, w0.y, c1 * h.y);
 }
 
+// The c1/c2 step with the rating entering the dot product's tree: `sel63`
+// is 1 in lane 63 and 0 elsewhere, so lane 63's partial starts from -r, its
+// tree total is the error (dot - r) itself, and one readlane brings it to the
+// scalar unit.
+template <class P, class A>
+__device__ __forceinline__ void sgd_step_b(P& w, P& h, A r, A sel63, A c1, A nstep) {
+    A p = fma_t(h.x, w.x, -r * sel63);
+    p = fma_t(h.y, w.y, p);
+    const A c2 = nstep * wave_tree_sum(p);
+    const P w0 = w;
+    w.x = fma_t(c2, h.x, c1 * w.x);
+    w.y = fma_t(c2, h.y, c1 * w.y);
+    h.x = fma_t(c2, w0.x, c1 * h.x);
+    h.y = fma_t(c2, w0.y, c1 * h.y);
+}
+
 // One SGD step on (w_i, h_j) given the rating r.  Per lane, elements x and y:
 //   p   = h.x*w.x ; p = fma(h.y, w.y, p) ; dot = tree(p) ; err = dot - r
 //   a   = fma(err, h, reg*w)     b = fma(err, w, reg*h)
@@ -438,5 +481,4 @@ __device__ __forceinline__ double readlane_f64(double x, int l) {
 }
 __device__ __forceinline__ float readlane_val(float x, int l) { return readlane_f32(x, l); }
 __device__ __forceinline__ double readlane_val(double x, int l) { return readlane_f64(x, l); }
-
 }  // namespace nomad_dev
