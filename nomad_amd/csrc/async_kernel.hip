@@ -168,7 +168,12 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
     const int npairs = a.ldf >> 1;
     const int64_t ldf = a.ldf;
     const uint32_t lkey = sr_lane_key(a.sr_seed, uint32_t(lane));
-    const uint32_t lkey_lo = lkey & 0xFFFFu, lkey_hi = lkey >> 16;
+    // Stochastic-rounding bits of the W stores: the lane's key plus a Weyl
+    // sequence of the wave (a scalar that advances by the golden-ratio
+    // constant per rating, started from a hash of seed, wave and launch), so
+    // a rating costs one vector add for its bits; the 16-bit halves of the
+    // sum are equidistributed per lane.  (Column stores keep the hashed bits.)
+    uint32_t sr_acc = mix32(a.sr_seed ^ mix32(uint32_t(g * NW + wave) * 0x9E3779B1u + uint32_t(a.epoch)));
     const int32_t* wb_g = a.wbound + int64_t(g) * NW;
     const int32_t row_lo = __builtin_amdgcn_readfirstlane(wb_g[wave]);
     const int32_t row_hi = __builtin_amdgcn_readfirstlane(wb_g[wave + 1]);
@@ -304,22 +309,18 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                 // inside the prefetch window (0: none).  Its prefetched copy
                 // predates that rating's update, so the row is taken from the
                 // ring of updated rows instead.
+                // (rows of one column are distinct: a single piece has none)
                 int32_t pdist = 0;
+                if (nt > 1) {
 #pragma unroll
-                for (int b = kPf - 1; b >= 1; --b) {
-                    const int32_t o = __shfl_up(srow, b);
-                    if (lane >= b && lane < total && o == srow) pdist = b;
+                    for (int b = kPf - 1; b >= 1; --b) {
+                        const int32_t o = __shfl_up(srow, b);
+                        if (lane >= b && lane < total && o == srow) pdist = b;
+                    }
                 }
                 // scalar mask over the stream's ratings: the first rating of each
                 // piece after the first
                 const uint64_t pmask = __ballot(lane > 0 && lane < total && __shfl_up(my_t, 1) != my_t);
-                // stochastic-rounding hash of each stream rating (lane u: its row
-                // and its piece's update key), one vector pass for the stream
-                uint32_t s_rh = 0;
-                if constexpr (RM == NOMAD_ROUND_STOCHASTIC) {
-                    const uint32_t srb_u = uint32_t(__shfl(int32_t(t_srb), my_t));
-                    s_rh = mix32(a.sr_seed ^ mix32(uint32_t(srow) * 0x9E3779B1u + srb_u));
-                }
 
                 int cur_t = 0;
                 int s_cur = readlane_i32(ts, 0);
@@ -363,9 +364,8 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                         w.x = pd != 0 ? fw.x : w.x;
                         w.y = pd != 0 ? fw.y : w.y;
                         sgd_step_b<P, Aa>(w, h, Aa(readlane_val(sval, u)), sel63, c1, nstep);
-                        // = sr_bits_k(a.sr_seed, row, key of the piece, lkey)
-                        const typename T::raw wr =
-                            T::template pack_k<RM>(w, uint32_t(readlane_i32(int32_t(s_rh), u)), lkey_lo, lkey_hi);
+                        sr_acc += 0x9E3779B9u;
+                        const typename T::raw wr = T::pack(w, lkey + sr_acc, RM);
                         T::store_raw_b(wrs, int(wvoff), wr, soff[d]);
                         sh.vic[wave][d][lane] = wr;
                         soff[d] = readlane_i32(soffN, u);
@@ -513,14 +513,6 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                     wp[d] = T::load_raw_b(wrs, int(wvoff), soff[d]);
                 }
             }
-            // stochastic-rounding hash of each chunk-A rating (as in the stream)
-            auto chunk_hash = [&](int32_t rows) -> uint32_t {
-                if constexpr (RM == NOMAD_ROUND_STOCHASTIC)
-                    return mix32(a.sr_seed ^ mix32(uint32_t(rows) * 0x9E3779B1u + sr_b));
-                else
-                    return 0u;
-            };
-            uint32_t rhA = chunk_hash(rA);
             const Aa nstep = -step;
             // piece ratings in chunk A: rows ascend, so they are a prefix
             int32_t cntA = __popcll(__ballot(rA < row_hi));
@@ -530,10 +522,8 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
             auto rating = [&](int uu, int d) {
                 P w = T::unpack(wp[d]);
                 sgd_step_b<P, Aa>(w, h, Aa(readlane_val(vA, uu)), sel63, c1, nstep);
-                // = sr_bits_k(a.sr_seed, row, sr_b, lkey)
-                T::store_raw_b(wrs, int(wvoff),
-                               T::template pack_k<RM>(w, uint32_t(readlane_i32(int32_t(rhA), uu)), lkey_lo, lkey_hi),
-                               soff[d]);
+                sr_acc += 0x9E3779B9u;
+                T::store_raw_b(wrs, int(wvoff), T::pack(w, lkey + sr_acc, RM), soff[d]);
                 soff[d] = readlane_i32(offN, uu);
                 wp[d] = T::load_raw_b(wrs, int(wvoff), soff[d]);
             };
@@ -548,7 +538,6 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                 // the piece goes on in the next chunk
                 rA = rB;
                 vA = vB;
-                rhA = chunk_hash(rA);
                 c0 += kWave;
                 load_chunk(c0 + kWave, rB, vB);
                 offN = ahead_offsets();

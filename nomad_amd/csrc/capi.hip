@@ -209,6 +209,7 @@ struct nomad_ctx {
     std::mt19937_64 rng_route;   // initial permutations (nb_main.cpp:79)
     std::mt19937_64 rng_reperm;  // re-permutation after a segment (nb_init.cpp:219, nb_main.cpp:316)
     int segments = 0;
+    int64_t async_launches = 0;  // launch index of the async kernel (salts its rounding bits)
     int64_t cycle_base = 0;      // route cycles completed by every active column
     int64_t strat_epochs = 0;
     // strat: per tile (row block b, column block cb) its non-empty column
@@ -740,6 +741,7 @@ int train_async(nomad_ctx* c, double seconds, int64_t epochs, nomad_train_stats*
     a.target_cycle = int32_t(std::min<int64_t>(target, INT_MAX));
     a.ncols_active = c->ncols_active;
     a.timing = (c->p.flags & NOMAD_FLAG_TIMING) ? 1 : 0;
+    a.epoch = int32_t(c->async_launches++ & 0x7FFFFFFF);
     const double limit = seconds > 0 ? seconds : watchdog_seconds();
     a.deadline = uint64_t(limit * c->wall_hz);
     hipError_t e = hipMemsetAsync(c->d_done, 0, 4, c->stream);

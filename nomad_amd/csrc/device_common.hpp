@@ -204,9 +204,6 @@ struct Store<NOMAD_STORE_FP32> {
     static __device__ __forceinline__ pair unpack(raw v) { return v; }
     // the stored form of a pair (rounding for bf16) and its store
     static __device__ __forceinline__ raw pack(pair p, uint32_t, int) { return p; }
-    // (the rounding key split as in Store<BF16>::pack_k; unused here)
-    template <int RM>
-    static __device__ __forceinline__ raw pack_k(pair p, uint32_t, uint32_t, uint32_t) { return p; }
     template <int CP = 0>
     static __device__ __forceinline__ void store_raw_b(__amdgpu_buffer_rsrc_t r, int voff, raw p, int soff = 0) {
         const u32x2_t v = {__float_as_uint(p.x), __float_as_uint(p.y)};
@@ -269,9 +266,6 @@ struct Store<NOMAD_STORE_FP64> {
     static __device__ __forceinline__ pair unpack(raw v) { return v; }
     // the stored form of a pair (rounding for bf16) and its store
     static __device__ __forceinline__ raw pack(pair p, uint32_t, int) { return p; }
-    // (the rounding key split as in Store<BF16>::pack_k; unused here)
-    template <int RM>
-    static __device__ __forceinline__ raw pack_k(pair p, uint32_t, uint32_t, uint32_t) { return p; }
     template <int CP = 0>
     static __device__ __forceinline__ void store_raw_b(__amdgpu_buffer_rsrc_t r, int voff, raw p, int soff = 0) {
         const uint64_t a = __builtin_bit_cast(uint64_t, p.x), b = __builtin_bit_cast(uint64_t, p.y);
@@ -350,19 +344,6 @@ struct Store<NOMAD_STORE_BF16> {
             return __builtin_amdgcn_perm(hi, lo, 0x07060302u);
         }
         return f32_to_bf16_rne(p.x) | (f32_to_bf16_rne(p.y) << 16);
-    }
-    // pack(p, s ^ key, round_mode) with the wave-uniform hash s and the lane
-    // key given as halves klo = key & 0xFFFF, khi = key >> 16: each element's
-    // rounding bits are (s's half ^ key's half), added in one xor-add.
-    template <int RM>
-    static __device__ __forceinline__ raw pack_k(pair p, uint32_t s, uint32_t klo, uint32_t khi) {
-        if constexpr (RM == NOMAD_ROUND_STOCHASTIC) {
-            const uint32_t lo = __float_as_uint(p.x) + ((s & 0xFFFFu) ^ klo);
-            const uint32_t hi = __float_as_uint(p.y) + ((s >> 16) ^ khi);
-            return __builtin_amdgcn_perm(hi, lo, 0x07060302u);
-        } else {
-            return f32_to_bf16_rne(p.x) | (f32_to_bf16_rne(p.y) << 16);
-        }
     }
     template <int CP = 0>
     static __device__ __forceinline__ void store_raw_b(__amdgpu_buffer_rsrc_t r, int voff, raw v, int soff = 0) {

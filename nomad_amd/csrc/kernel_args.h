@@ -73,7 +73,7 @@ struct SgdArgs {
     // strat schedule
     const int64_t* tile_ptr;   // B * B + 1: first visit of tile (b, cb) at [b * B + cb] in visits
     int32_t substep;
-    int32_t epoch;
+    int32_t epoch;             // strat: the epoch; async: the launch index
 };
 
 // Batched stratified launch: at most kBatchMax problems per launch.
