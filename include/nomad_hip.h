@@ -50,8 +50,8 @@ extern "C" {
 #define NOMAD_ROUND_STOCHASTIC 1
 
 /* nomad_params.flags */
-#define NOMAD_FLAG_BALANCE 1 /* async: row blocks hold about nnz / P training
-                                ratings each instead of ceil(m / P) rows     */
+#define NOMAD_FLAG_BALANCE 1 /* async: row blocks balanced by training ratings
+                                (nomad_row_blocks) instead of ceil(m / P) rows */
 #define NOMAD_FLAG_TIMING 2  /* async: collect the wall-clock counters [8..13]
                                 of nomad_get_counters (slows the kernel)    */
 
@@ -198,6 +198,17 @@ int nomad_autotune_batch(const nomad_batch_problem* probs, int32_t count, int32_
  * FP64 (the dot product runs in the storage's arithmetic type). */
 int nomad_eval(nomad_ctx* ctx, nomad_eval_stats* stats);
 
+/* Row blocks of NOMAD_FLAG_BALANCE: bounds[0..P] (bounds[0] = 0, bounds[P] =
+ * m) split rows 0..m-1 into P contiguous non-empty blocks whose largest
+ * training-rating count relative to the block's capacity cap[b] (in (0, 1];
+ * null: all 1) is the smallest any contiguous split allows (with all
+ * capacities 1 exactly; otherwise unless a block had to be split to reach P
+ * blocks).  Replaces the reference's equal-row split of the rows over
+ * machines and threads (Code/nomad/nb_data.cpp:83-89) when the flag is set;
+ * host code only (no device is touched).  Needs 0 < P <= m. */
+int nomad_row_blocks(const int32_t* rowidx, int64_t nnz, int32_t m, int32_t P, const double* cap,
+                     int32_t* bounds);
+
 /* Number of workers / strata the context actually uses (after auto sizing). */
 int nomad_get_nworkers(nomad_ctx* ctx, int32_t* nworkers);
 
@@ -214,7 +225,7 @@ int nomad_get_nworkers(nomad_ctx* ctx, int32_t* nworkers);
  * items (streams of pieces, or single long pieces) a wave ran.  Counters
  * [8..13] are collected only with NOMAD_FLAG_TIMING.  Ticks run at the
  * device's constant wall-clock rate.  When n exceeds 16, out[16 ..]
- * receives the same 16 counters for each workgroup in turn, as far as n
+ * receives the same 16 counters for each worker (wave) in turn, as far as n
  * allows.  n is the capacity of out. */
 int nomad_get_counters(nomad_ctx* ctx, int64_t* out, int32_t n);
 

@@ -793,7 +793,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
         tk_inbox_l += __shfl_down(tk_inbox_l, off, 64);
     }
     if (lane == 0) {
-        int64_t* st = a.stats + int64_t(g) * nomad_k::kStatWidth;
+        int64_t* st = a.stats + int64_t(g * NW + wave) * nomad_k::kStatWidth;  // per worker
         const int64_t v[16] = {n_upd, n_vis, n_idle, n_fetch, n_fetched, n_exit, n_res, n_blocked,
                                tk_visit, tk_exit, tk_fetch, tk_chunk, tk_transit_l, tk_inbox_l, n_depth, n_work};
         for (int i = 0; i < 16; ++i)

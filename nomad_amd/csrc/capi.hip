@@ -159,6 +159,90 @@ namespace nomad_host {
 int fail(int code, const std::string& msg) { return ::fail(code, msg); }
 }  // namespace nomad_host
 
+// Contiguous row blocks minimising the largest block's rating count relative
+// to its capacity cap[b] (all 1 when cap is null): the smallest bound L for
+// which closing block b before the row that would take it past L * cap[b]
+// needs at most P blocks (a row heavier than L * cap[b] gets a block of its
+// own), found by bisection; if that split has fewer than P blocks, the block
+// with the most rows is halved (by ratings) until there are P, so no block
+// is empty.  O(nnz + m log nnz) host time.
+extern "C" int nomad_row_blocks(const int32_t* rowidx, int64_t nnz, int32_t m, int32_t P, const double* cap,
+                                int32_t* bounds) {
+    if (!bounds || m <= 0 || P <= 0 || P > m || nnz < 0 || (nnz > 0 && !rowidx))
+        return fail(NOMAD_EINVAL, "nomad_row_blocks: need 0 < P <= m, nnz >= 0 and the row indices");
+    if (cap)
+        for (int32_t b = 0; b < P; ++b)
+            if (!(cap[b] > 0 && cap[b] <= 1)) return fail(NOMAD_EINVAL, "nomad_row_blocks: capacities must be in (0, 1]");
+    auto limit = [&](int64_t L, int64_t b) -> double { return cap && b < P ? double(L) * cap[b] : double(L); };
+    std::vector<int64_t> deg(size_t(m), 0);
+    for (int64_t e = 0; e < nnz; ++e) {
+        if (rowidx[e] < 0 || rowidx[e] >= m) return fail(NOMAD_EINVAL, "nomad_row_blocks: row index out of range");
+        ++deg[size_t(rowidx[e])];
+    }
+    auto blocks_for = [&](int64_t L) {
+        int64_t k = 1, cur = 0;
+        for (int32_t r = 0; r < m; ++r) {
+            if (cur > 0 && double(cur + deg[size_t(r)]) > limit(L, k - 1)) {
+                ++k;
+                cur = 0;
+            }
+            cur += deg[size_t(r)];
+        }
+        return k;
+    };
+    double cmin = 1.0;
+    if (cap)
+        for (int32_t b = 0; b < P; ++b) cmin = std::min(cmin, cap[b]);
+    int64_t lo = (nnz + P - 1) / P, hi = std::max<int64_t>(lo, int64_t(double(std::max<int64_t>(nnz, 1)) / cmin) + 1);
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (blocks_for(mid) <= P)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    std::vector<int32_t> start{0};
+    int64_t cur = 0;
+    for (int32_t r = 0; r < m; ++r) {
+        if (cur > 0 && double(cur + deg[size_t(r)]) > limit(lo, int64_t(start.size()) - 1)) {
+            start.push_back(r);
+            cur = 0;
+        }
+        cur += deg[size_t(r)];
+    }
+    // fewer than P blocks: halve the block with the most rows, by ratings
+    // (every split leaves at least one row on each side), largest first
+    if (int32_t(start.size()) < P) {
+        std::vector<std::pair<int32_t, int32_t>> heap;  // (rows, first row), max-heap on rows
+        for (size_t b = 0; b < start.size(); ++b)
+            heap.push_back({(b + 1 < start.size() ? start[b + 1] : m) - start[b], start[b]});
+        std::make_heap(heap.begin(), heap.end());
+        int32_t nblk = int32_t(heap.size());
+        while (nblk < P) {
+            std::pop_heap(heap.begin(), heap.end());
+            const auto [rows, b0] = heap.back();
+            heap.pop_back();
+            const int32_t b1 = b0 + rows;  // rows >= 2: P <= m and every block has a row
+            int64_t tot = 0;
+            for (int32_t r = b0; r < b1; ++r) tot += deg[size_t(r)];
+            int32_t cut = b0 + 1;
+            int64_t acc = deg[size_t(b0)];
+            while (cut < b1 - 1 && 2 * (acc + deg[size_t(cut)]) <= tot) acc += deg[size_t(cut++)];
+            heap.push_back({cut - b0, b0});
+            std::push_heap(heap.begin(), heap.end());
+            heap.push_back({b1 - cut, cut});
+            std::push_heap(heap.begin(), heap.end());
+            ++nblk;
+        }
+        start.clear();
+        for (const auto& blk : heap) start.push_back(blk.second);
+        std::sort(start.begin(), start.end());
+    }
+    for (int32_t b = 0; b < P; ++b) bounds[b] = start[size_t(b)];
+    bounds[P] = m;
+    return NOMAD_OK;
+}
+
 struct nomad_ctx {
     nomad_params p{};
     int P = 0;  // row blocks: workers (async) or strata (strat)
@@ -373,27 +457,23 @@ int ensure_tiles(nomad_ctx* c) {
 
 // Row blocks of the async workers.  Default: ceil(m / P) rows each, the
 // reference's split (Code/nomad/nb_data.cpp:83-89).  With NOMAD_FLAG_BALANCE
-// the blocks are contiguous row ranges holding about nnz / P training ratings
-// each (block b starts at the first row whose rating prefix sum reaches
-// b * nnz / P).
+// the blocks are contiguous row ranges whose largest training-rating count,
+// relative to the block's capacity, is as small as a contiguous split into P
+// blocks allows (nomad_row_blocks).
 void build_partition(nomad_ctx* c) {
     const int32_t m = c->p.m;
     const int P = c->P;
     c->h_wbound.assign(size_t(P) + 1, m);
     if (c->p.flags & NOMAD_FLAG_BALANCE) {
-        std::vector<int64_t> deg(size_t(m) + 1, 0);
-        for (int32_t r : c->h_rowidx) deg[size_t(r) + 1]++;
-        for (int32_t r = 0; r < m; ++r) deg[size_t(r) + 1] += deg[size_t(r)];
-        const int64_t nnz = deg[size_t(m)];
-        int32_t r = 0;
-        for (int b = 0; b <= P; ++b) {
-            // first row r with prefix(r) >= ceil(b * nnz / P)
-            const int64_t want = (int64_t(b) * nnz + P - 1) / P;
-            while (r < m && deg[size_t(r)] < want) ++r;
-            c->h_wbound[size_t(b)] = r;
-        }
-        c->h_wbound[0] = 0;
-        c->h_wbound[size_t(P)] = m;
+        // The last wave of a workgroup hands every column on to the next
+        // workgroup; those exits take about 0.37x the rest of its work at 8
+        // waves per workgroup (device counters, Netflix shape), 0.046 x waves
+        // in general, so its block's capacity is 1 / (1 + 0.046 x waves).
+        std::vector<double> cap(size_t(P), 1.0);
+        const int nw = c->nw > 0 ? c->nw : 1;
+        if (nw > 1)
+            for (int b = nw - 1; b < P; b += nw) cap[size_t(b)] = 1.0 / (1.0 + 0.046 * nw);
+        nomad_row_blocks(c->h_rowidx.data(), int64_t(c->h_rowidx.size()), m, P, cap.data(), c->h_wbound.data());
     } else {
         for (int b = 0; b <= P; ++b) c->h_wbound[size_t(b)] = int32_t(std::min<int64_t>(int64_t(b) * c->rpb, m));
     }
@@ -533,12 +613,14 @@ int setup_async(nomad_ctx* c) {
         if ((rc = coh_alloc(c->p.device, &c->d_qhead, size_t(P) * nomad_k::kQStride))) return rc;
         if ((rc = coh_alloc(c->p.device, &c->d_done, 1))) return rc;
         if ((rc = coh_alloc(c->p.device, &c->d_timed_out, 1))) return rc;
-        if ((rc = dev_alloc(&c->d_stats, size_t(P) * kStatWidth))) return rc;
+        // device counters: one record per worker (wave)
+        const size_t nstat = size_t(P) * size_t(c->nw) * kStatWidth;
+        if ((rc = dev_alloc(&c->d_stats, nstat))) return rc;
         NOMAD_HIP_TRY(hipMemset(c->d_qslot, 0, (size_t(P) << c->qlog2) * 8));
         NOMAD_HIP_TRY(hipMemset(c->d_qtail, 0, size_t(P) * nomad_k::kQStride * 4));
         NOMAD_HIP_TRY(hipMemset(c->d_qhead, 0, size_t(P) * nomad_k::kQStride * 4));
-        NOMAD_HIP_TRY(hipMemset(c->d_stats, 0, size_t(P) * kStatWidth * 8));
-        c->stats_prev.assign(size_t(P) * kStatWidth, 0);
+        NOMAD_HIP_TRY(hipMemset(c->d_stats, 0, nstat * 8));
+        c->stats_prev.assign(nstat, 0);
         for (auto& t : tok) t = TokInfo{0, 0, 0, 0};
         c->routes_ready = true;
         c->segments = 0;
@@ -765,10 +847,10 @@ int train_async(nomad_ctx* c, double seconds, int64_t epochs, nomad_train_stats*
     uint32_t done = 0, timed_out = 0;
     NOMAD_HIP_TRY(hipMemcpy(&done, c->d_done, 4, hipMemcpyDeviceToHost));
     NOMAD_HIP_TRY(hipMemcpy(&timed_out, c->d_timed_out, 4, hipMemcpyDeviceToHost));
-    std::vector<int64_t> stats(size_t(P) * kStatWidth);
+    std::vector<int64_t> stats(size_t(P) * size_t(c->nw) * kStatWidth);  // per worker
     NOMAD_HIP_TRY(hipMemcpy(stats.data(), c->d_stats, stats.size() * 8, hipMemcpyDeviceToHost));
     int64_t sum[kStatWidth] = {0};
-    for (int32_t q = 0; q < P; ++q)
+    for (int32_t q = 0; q < P * c->nw; ++q)
         for (int i = 0; i < kStatWidth; ++i) sum[i] += stats[size_t(q) * kStatWidth + i] - c->stats_prev[size_t(q) * kStatWidth + i];
     c->last_unit_stats.resize(stats.size());
     for (size_t i = 0; i < stats.size(); ++i) c->last_unit_stats[i] = stats[i] - c->stats_prev[i];

@@ -110,6 +110,7 @@ HIP_SYMBOLS = {
     "nomad_train": (C.c_int, [_P, C.c_double, C.c_int64, C.POINTER(TrainStats)]),
     "nomad_eval": (C.c_int, [_P, C.POINTER(EvalStats)]),
     "nomad_get_nworkers": (C.c_int, [_P, _i32p]),
+    "nomad_row_blocks": (C.c_int, [_i32p, C.c_int64, C.c_int32, C.c_int32, _f64p, _i32p]),
     "nomad_get_counters": (C.c_int, [_P, _i64p, C.c_int32]),
     "nomad_train_batch": (C.c_int, [C.POINTER(_P), C.c_int32, C.c_int64, C.c_int32, C.POINTER(TrainStats)]),
     "nomad_autotune": (C.c_int, [C.POINTER(Params), _i64p, _i32p, _f64p, C.c_int64, C.c_double, C.POINTER(TuneTrial),
@@ -268,6 +269,19 @@ def synth(spec: SynthSpec) -> tuple[Csc, Csc]:
         L.nomad_tools_matrix_free(b)
 
 
+def row_blocks(rowidx, m: int, P: int, cap=None) -> np.ndarray:
+    """First rows of the P balanced row blocks (NOMAD_FLAG_BALANCE), plus m;
+    cap: optional per-block capacities in (0, 1].  Host only."""
+    r = np.ascontiguousarray(rowidx, np.int32)
+    out = np.zeros(P + 1, np.int32)
+    cp = None if cap is None else np.ascontiguousarray(cap, np.float64)
+    rc = hip().nomad_row_blocks(_ptr(r, C.c_int32), r.size, m, P, None if cp is None else _ptr(cp, C.c_double),
+                                _ptr(out, C.c_int32))
+    if rc != 0:
+        raise NomadError(rc, hip().nomad_last_error().decode())
+    return out
+
+
 def init_factors(m: int, n: int, k: int, nthreads: int, seed: int):
     W = np.zeros((m, k), np.float64)
     H = np.zeros((n, k), np.float64)
@@ -419,7 +433,7 @@ class Context:
         return dict(zip(self.COUNTER_NAMES, out.tolist()))
 
     def unit_counters(self, units: int) -> np.ndarray:
-        """The same counters per workgroup: array (units, 16)."""
+        """The same counters per worker (wave): array (units, 16)."""
         out = np.zeros(16 * (units + 1), np.int64)
         self._check(self._lib.nomad_get_counters(self._h, _ptr(out, C.c_int64), out.size))
         return out[16:].reshape(units, 16)
