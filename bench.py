@@ -63,6 +63,22 @@ def make_ctx(lib, tr, te, W0, H0, *, storage, rounding, k, hyp, nworkers):
     return ctx
 
 
+def interp_hit(rmse0, times, curve, target):
+    """Device time at which the test RMSE first reaches target, linearly
+    interpolated between the two evaluations around the crossing (the curve
+    starts at rmse0 at time 0); None if it is never reached."""
+    if target is None:
+        return None
+    if rmse0 <= target:
+        return 0.0
+    t_prev, r_prev = 0.0, rmse0
+    for t, r in zip(times, curve):
+        if r <= target:
+            return round(t_prev + (r_prev - target) / (r_prev - r) * (t - t_prev), 4)
+        t_prev, r_prev = t, r
+    return None
+
+
 def run_path(lib, tr, te, W0, H0, *, storage, rounding, k, hyp, nworkers, warmup, steps, segment, target=None):
     if warmup > 0:
         ctx = make_ctx(lib, tr, te, W0, H0, storage=storage, rounding=rounding, k=k, hyp=hyp, nworkers=nworkers)
@@ -75,6 +91,8 @@ def run_path(lib, tr, te, W0, H0, *, storage, rounding, k, hyp, nworkers, warmup
     dev_s = wall_s = alg = 0.0
     visits = 0
     t_hit = None
+    ev0 = ctx.eval()
+    rmse0 = float(np.sqrt(ev0.test_sse / max(ev0.test_count, 1)))  # before training
     for step in range(steps):
         t0 = time.perf_counter()
         st = ctx.train(seconds=segment)
@@ -94,6 +112,7 @@ def run_path(lib, tr, te, W0, H0, *, storage, rounding, k, hyp, nworkers, warmup
     nw = ctx.nworkers
     ctx.close()
     return dict(updates=upd, dev_s=dev_s, wall_s=wall_s, alg_bytes=alg, visits=visits, curve=curve, times=times,
+                rmse0=rmse0,
                 t_hit=t_hit, nworkers=nw)
 
 
@@ -412,6 +431,9 @@ def main() -> int:
     if a.only in (None, "bf16"):
         res["bf16"] = run_path(lib, tr, te, W0, H0, storage="bf16", rounding="stochastic", target=target, **common)
 
+    for r in res.values():
+        r["t_hit_interp"] = interp_hit(r["rmse0"], r["times"], r["curve"], target)
+
     head_name = "bf16" if "bf16" in res else "fp32"
     head = res[head_name]
     value = head["updates"] / head["dev_s"]
@@ -456,6 +478,7 @@ def main() -> int:
                 "train_seconds": r["times"],
                 "test_rmse_curve": r["curve"],
                 "time_to_target_s": r["t_hit"],
+                "time_to_target_interp_s": r["t_hit_interp"],
             } for name, r in res.items()
         },
         "target_test_rmse": target,
