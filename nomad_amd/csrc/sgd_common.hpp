@@ -42,13 +42,15 @@ __device__ __forceinline__ void row_store(typename Store<S>::elem* row, int lane
 // lane l's pair of row r sits at byte offset wvoff + (r - rlo) * rowbytes,
 // wvoff = l * pair bytes (or 2^31 for lanes past the row, so that they are
 // out of range), and kDrop puts every lane out of range (a store the range
-// check drops).
+// check drops).  The row part (r - rlo) * rowbytes goes to the scalar offset
+// of a load or store (it is always a row of the block; the hardware range
+// check covers the per-lane offset only).
 struct RowBlock {
     __amdgpu_buffer_rsrc_t rsrc;
     uint32_t wvoff;
     int32_t rlo;
     int32_t rowbytes;
-    __device__ __forceinline__ int off(int32_t r) const { return int(wvoff + uint32_t(r - rlo) * uint32_t(rowbytes)); }
+    __device__ __forceinline__ int soff(int32_t r) const { return (r - rlo) * rowbytes; }
     __device__ __forceinline__ int drop(int d = 0) const { return int(wvoff + 0x7FFFFFFFu) - d * 1024; }
 };
 
@@ -94,7 +96,7 @@ __device__ __forceinline__ void run_chunk(const SgdArgs& a, const RowBlock& rb, 
 #pragma unroll
     for (int d = 0; d < kD; ++d) {
         T::template store_raw_b<CP>(rb.rsrc, rb.drop(d), T::pack(P{Aa(0), Aa(0)}, 0u, 0));
-        wp[d] = T::template load_raw_b<CP>(rb.rsrc, rb.off(readlane_i32(myrow, min(d, cnt - 1))));
+        wp[d] = T::template load_raw_b<CP>(rb.rsrc, int(rb.wvoff), rb.soff(readlane_i32(myrow, min(d, cnt - 1))));
     }
     // groups of kG ratings; a group that starts past the chunk leaves both
     // loops at once (an exit edge keeps the loop's wait counts exact)
@@ -116,8 +118,10 @@ __device__ __forceinline__ void run_chunk(const SgdArgs& a, const RowBlock& rb, 
                 h.x = real ? h.x : h0.x;
                 h.y = real ? h.y : h0.y;
                 const uint32_t rnd = sr_bits_k(a.sr_seed, uint32_t(i), sr_b, lkey);
-                T::template store_raw_b<CP>(rb.rsrc, real ? rb.off(i) : rb.drop(), T::pack(w, rnd, RM));
-                wp[d] = T::template load_raw_b<CP>(rb.rsrc, rb.off(readlane_i32(myrow, min(t + kD, cnt - 1))));
+                T::template store_raw_b<CP>(rb.rsrc, real ? int(rb.wvoff) : rb.drop(), T::pack(w, rnd, RM),
+                                            real ? rb.soff(i) : 0);
+                wp[d] = T::template load_raw_b<CP>(rb.rsrc, int(rb.wvoff),
+                                                   rb.soff(readlane_i32(myrow, min(t + kD, cnt - 1))));
             }
         }
     }
