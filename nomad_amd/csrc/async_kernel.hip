@@ -310,13 +310,17 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                 // predates that rating's update, so the row is taken from the
                 // ring of updated rows instead.
                 // (rows of one column are distinct: a single piece has none)
+                // (all shuffles first, into registers of their own, so that they
+                // are in flight together rather than one LDS round trip each)
                 int32_t pdist = 0;
                 if (nt > 1) {
+                    int32_t prev[kPf];
 #pragma unroll
-                    for (int b = kPf - 1; b >= 1; --b) {
-                        const int32_t o = __shfl_up(srow, b);
-                        if (lane >= b && lane < total && o == srow) pdist = b;
-                    }
+                    for (int b = 1; b < kPf; ++b) prev[b] = __shfl_up(srow, b);
+                    __builtin_amdgcn_sched_barrier(0);  // (keeps the compiler from interleaving them)
+#pragma unroll
+                    for (int b = kPf - 1; b >= 1; --b)
+                        if (lane >= b && lane < total && prev[b] == srow) pdist = b;
                 }
                 // scalar mask over the stream's ratings: the first rating of each
                 // piece after the first
