@@ -386,25 +386,11 @@ struct Store<NOMAD_STORE_BF16> {
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
-// The same step written as w' = c1 w + c2 h, h' = c1 h + c2 w (old w) with
-// c1 = 1 - step * reg fixed per visit and c2 = -step * err: four operations
-// per element instead of six (one rounding less), for the asynchronous
-// kernel, whose results are compared by RMSE and not bit for bit.  With
-// c1 = 1 and nstep = 0 both factors come out exactly unchanged (idle slots).
-template <class P, class A>
-__device__ __forceinline__ void sgd_step_c(P& w, P& h, A r, A c1, A nstep) {
-    A p = h.x * w.x;
-    p = fma_t(h.y, w.y, p);
-    const A dot = wave_tree_sum(p);
-    const A c2 = nstep * (dot - r);
-    const P w0 = w;
-    w.x = fma_t(c2, h.x, c1 * w.x);
-    w.y = fma_t(c2, h.y, c1 * w.y);
-    h.x = fma_t(c2, w0.x, c1 * h.x);
-    h.y = fma_t(c2, w0.y, c1 * h.y);
-}
-
-// The c1/c2 step with the rating entering the dot product's tree: `sel63`
+// The asynchronous kernel's step, written as w' = c1 w + c2 h,
+// h' = c1 h + c2 w (old w) with c1 = 1 - step * reg fixed per visit and
+// c2 = -step * err: four operations per element instead of six (one
+// rounding less); its results are compared by RMSE and not bit for bit.
+// The rating enters the dot product's tree: `sel63`
 // is 1 in lane 63 and 0 elsewhere, so lane 63's partial starts from -r, its
 // tree total is the error (dot - r) itself, and one readlane brings it to the
 // scalar unit.
