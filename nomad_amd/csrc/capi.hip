@@ -697,23 +697,23 @@ hipError_t batch_occupancy(int* per_cu) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, sgd_batch_kernel<S>, 256, 0);
 }
 
-template <int S>
-constexpr int async_slots() {
-    return S == NOMAD_STORE_FP64 ? 32 : 64;
-}
+using nomad_k::async_slots;
 
 // async kernel instances: rounding is a template argument for bf16 only
 template <int S, int RM>
 hipError_t launch_async_rm(const SgdArgs& a, int nw, hipStream_t st) {
-    constexpr int slots = async_slots<S>();
     switch (nw) {
         case 4:
-            hipLaunchKernelGGL((sgd_async_kernel<S, 4, slots, RM>), dim3(a.nblocks), dim3(4 * 64), 0, st, a);
+            hipLaunchKernelGGL((sgd_async_kernel<S, 4, async_slots<S, 4>(), RM>), dim3(a.nblocks), dim3(4 * 64), 0,
+                               st, a);
             break;
         case 8:
-            hipLaunchKernelGGL((sgd_async_kernel<S, 8, slots, RM>), dim3(a.nblocks), dim3(8 * 64), 0, st, a);
+            hipLaunchKernelGGL((sgd_async_kernel<S, 8, async_slots<S, 8>(), RM>), dim3(a.nblocks), dim3(8 * 64), 0,
+                               st, a);
             break;
-        default: hipLaunchKernelGGL((sgd_async_kernel<S, 16, slots, RM>), dim3(a.nblocks), dim3(16 * 64), 0, st, a);
+        default:
+            hipLaunchKernelGGL((sgd_async_kernel<S, 16, async_slots<S, 16>(), RM>), dim3(a.nblocks),
+                               dim3(16 * 64), 0, st, a);
     }
     return hipGetLastError();
 }
@@ -728,15 +728,16 @@ hipError_t launch_async(const SgdArgs& a, int nw, int rounding, hipStream_t st) 
 
 template <int S, int RM>
 hipError_t async_occupancy_rm(int nw, int* per_cu) {
-    constexpr int slots = async_slots<S>();
     switch (nw) {
         case 4:
-            return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, sgd_async_kernel<S, 4, slots, RM>, 4 * 64, 0);
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, sgd_async_kernel<S, 4, async_slots<S, 4>(), RM>,
+                                                                4 * 64, 0);
         case 8:
-            return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, sgd_async_kernel<S, 8, slots, RM>, 8 * 64, 0);
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, sgd_async_kernel<S, 8, async_slots<S, 8>(), RM>,
+                                                                8 * 64, 0);
         default:
-            return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, sgd_async_kernel<S, 16, slots, RM>, 16 * 64,
-                                                                0);
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                per_cu, sgd_async_kernel<S, 16, async_slots<S, 16>(), RM>, 16 * 64, 0);
     }
 }
 
