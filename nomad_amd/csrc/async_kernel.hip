@@ -47,12 +47,14 @@ constexpr int kFetchBatch = 16;  // columns claimed from the ring per fetch
 constexpr int kBatch = 16;       // inbox slots examined per stream
 // Rows of W in flight inside a stream, and updated rows kept for forwarding
 // (per wave, in LDS: a ring of as many rows as this prefetch distance).  At
-// 8 waves 8 rows: faster than 16 for fp32 (3.67e9 against 3.59e9 updates/s,
-// Netflix shape) and the room that lets bf16 hold 128 token slots; else the
-// rings of a workgroup take 64 KiB.
+// 8 waves 4 rows for fp32 and bf16: the room that lets a workgroup hold 128
+// token slots, and no slower than 8 or 16 there (bf16, Netflix shape: 4.18e9
+// updates/s against 4.12e9 with 8); otherwise the rings of a workgroup take
+// 64 KiB.
 template <int S, int NW>
 constexpr int stream_depth() {
-    return NW == 8 ? 8 : (64 * 1024) / (NW * kWave * 2 * int(sizeof(typename Store<S>::acc)));
+    return (NW == 8 && S != NOMAD_STORE_FP64) ? 4
+                                              : (64 * 1024) / (NW * kWave * 2 * int(sizeof(typename Store<S>::acc)));
 }
 
 // W rows in flight inside one long piece: deep enough to cover HBM latency

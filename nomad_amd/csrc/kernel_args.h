@@ -30,13 +30,13 @@ struct alignas(16) TokInfo {
 constexpr int kQStride = 32;
 
 // LDS token slots per workgroup of the async kernel (storage S = NOMAD_STORE_*
-// 0 fp32, 1 bf16, 2 fp64; NW waves): 128 for bf16 at 8 waves, whose packed
-// rows and 8-deep stream prefetch leave room for them in the 160 KiB, 32 for
-// fp64, 64 otherwise.  More resident columns keep a workgroup's waves busy:
-// bf16 at 8 waves ran 19 % faster with 128 slots than with 64.
+// 0 fp32, 1 bf16, 2 fp64; NW waves): 128 for fp32 and bf16 at 8 waves, where
+// a 4-deep stream prefetch leaves room for them in the 160 KiB, 32 for fp64,
+// 64 otherwise.  More resident columns keep a workgroup's waves busy: at 8
+// waves bf16 ran 19 % and fp32 17 % faster with 128 slots than with 64.
 template <int S, int NW>
 constexpr int async_slots() {
-    return S == 2 ? 32 : (S == 1 && NW == 8) ? 128 : 64;
+    return S == 2 ? 32 : NW == 8 ? 128 : 64;
 }
 
 // per-workgroup counters of the async kernel
