@@ -138,6 +138,30 @@ def test_async_rmse_matches_cpu_reference(gpu, small):
     assert abs(gpu_rmse - cpu_rmse) <= 0.005 * cpu_rmse, (gpu_rmse, cpu_rmse)
 
 
+@pytest.mark.parametrize("storage", ["fp32", "bf16"])
+def test_async_eight_waves_per_workgroup(gpu, small, storage):
+    """1,536 workers run as 192 workgroups of 8 waves, the configuration with
+    128 LDS token slots and a 4-deep stream prefetch (the benchmark's): exact
+    update counts every epoch, and after 8 epochs the test RMSE within 0.5 %
+    of the CPU async reference."""
+    tr, te = small
+    k = 16
+    epochs = 8
+    W0, H0 = gpu.init_factors(tr.m, tr.n, k, 4, 12345)
+    rnd = "stochastic" if storage == "bf16" else "nearest"
+    with _ctx(gpu, tr, te, k, storage=storage, rounding=rnd, mode="async", nworkers=1536) as ctx:
+        assert ctx.nworkers == 1536
+        ctx.set_factors(W0, H0)
+        for ep in range(epochs):
+            st = ctx.train(epochs=1)
+            assert st.updates == tr.nnz and st.epochs_done == ep + 1, (ep, st.updates)
+        ev = ctx.eval()
+    gpu_rmse = np.sqrt(ev.test_sse / ev.test_count)
+    Wc, Hc, _ = O.sgd_async(tr, W0, H0, storage=O.FP32, T=4, seed=12345, epochs=epochs, **HYP)
+    cpu_rmse = np.sqrt(O.sse(te, Wc, Hc, storage=O.FP32) / te.nnz)
+    assert abs(gpu_rmse - cpu_rmse) <= 0.005 * cpu_rmse, (gpu_rmse, cpu_rmse)
+
+
 def test_async_timeout_segments(gpu, small):
     tr, te = small
     k = 16
