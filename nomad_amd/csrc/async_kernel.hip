@@ -435,14 +435,20 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                             if (timing) st_dc(a.tok_stamp + j, uint64_t(wall_clock64()));
                         }
                     }
+                    // the ring positions are reserved while the stores drain (a
+                    // reserved entry is not consumed before it is published)
+                    int32_t q = 0;
+                    uint32_t p = 0;
+                    if (t_exit && !t_park) {
+                        q = sh.nwg[ts];
+                        p = __hip_atomic_fetch_add(a.qtail + int64_t(q) * nomad_k::kQStride, 1u, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                    }
                     wait_all_mem();
                     if (t_exit) {
                         if (t_park) {
                             __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         } else {
-                            const int32_t q = sh.nwg[ts];
-                            const uint32_t p = __hip_atomic_fetch_add(a.qtail + int64_t(q) * nomad_k::kQStride, 1u,
-                                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                             const uint64_t v = (uint64_t((p >> a.qlog2) + 1u) << 32) | uint32_t(t_j);
                             st_dc(a.qslot + (int64_t(q) << a.qlog2) + (p & gmask), v);
                         }
@@ -609,14 +615,14 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                     if (lane == 0) __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 } else {
                     const int32_t q = sh.nwg[s];
-                    // h_j and TokInfo complete before a ring position is even
-                    // reserved, so the consumer never waits long on a reserved
-                    // but unpublished entry
                     if (timing && lane == 0) st_dc(a.tok_stamp + j, uint64_t(wall_clock64()));
+                    // the ring position is reserved while the stores drain
+                    uint32_t t = 0;
+                    if (lane == 0)
+                        t = __hip_atomic_fetch_add(a.qtail + int64_t(q) * nomad_k::kQStride, 1u, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
                     wait_all_mem();
                     if (lane == 0) {
-                        const uint32_t t = __hip_atomic_fetch_add(a.qtail + int64_t(q) * nomad_k::kQStride, 1u,
-                                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         const uint64_t v = (uint64_t((t >> a.qlog2) + 1u) << 32) | uint32_t(j);
                         st_dc(a.qslot + (int64_t(q) << a.qlog2) + (t & gmask), v);
                     }
