@@ -653,12 +653,8 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
             // reads its result back.  (Plain reads of words another lane just
             // changed are data races the compiler may resolve per lane.)
             const uint32_t gh = __builtin_amdgcn_readfirstlane(lds_ld(&sh.ghead));
-            // lane 0 looks at the head entry; the batch is read only when that
-            // one is published
-            uint64_t hv = 0;
-            if (lane == 0) hv = ld_dc(ring + (gh & gmask));
-            hv = uniform64(hv);
-            if (uint32_t(hv >> 32) == ((gh >> a.qlog2) + 1u)) {
+            // the batch's entries in one round trip (the lock is held meanwhile)
+            {
                 bool valid = false;
                 if (lane < kFetchBatch) {
                     sv = ld_dc(ring + ((gh + uint32_t(lane)) & gmask));
