@@ -59,10 +59,10 @@ constexpr int stream_depth() {
 
 // W rows in flight inside one long piece: deep enough to cover HBM latency
 // with one wave per SIMD (4 waves per workgroup), less where more waves share
-// the SIMD's registers
+// the SIMD (at 8 waves 8 rows were as fast as 16)
 template <int NW>
 constexpr int long_prefetch_depth() {
-    return NW <= 4 ? 32 : NW <= 8 ? 16 : 8;
+    return NW <= 4 ? 32 : 8;
 }
 
 template <int S, int NW, int SLOTS>
