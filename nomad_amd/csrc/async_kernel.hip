@@ -375,7 +375,10 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                         sgd_step_b<P, Aa>(w, h, Aa(readlane_val(sval, u)), sel63, c1, nstep);
                         sr_acc += 0x9E3779B9u;
                         const typename T::raw wr = T::pack(w, lkey + sr_acc, RM);
-                        T::store_raw_b(wrs, int(wvoff), wr, soff[d]);
+                        // (the offset is uniform; for bf16 the compiler carries some
+                        // soff[d] across the loop in a vector register, and without
+                        // the readfirstlane it wraps the store in a per-value loop)
+                        T::store_raw_b(wrs, int(wvoff), wr, __builtin_amdgcn_readfirstlane(soff[d]));
                         sh.vic[wave][d][lane] = wr;
                         soff[d] = readlane_i32(soffN, u);
                         wp[d] = T::load_raw_b(wrs, int(wvoff), soff[d]);
