@@ -11,11 +11,12 @@ resident in HBM, then training stops and the test RMSE is evaluated.  This is
 the reference's timeout schedule (Code/nomad/nb_main.cpp:109-134: train until
 the next timeout, stop, evaluate), and updates/s is its u= count over training
 time, evaluation excluded (nb_main.cpp:265-266, 284).  The FP32-stored path
-runs first; its final test RMSE x 1.005 is the target.  The BF16-stored path
-(stochastic rounding, FP32 arithmetic) then runs the same schedule on a fresh
-context, and the training time at which it first reaches the target is
-reported.  `value` is the BF16 path's updates/s.  Warmup segments run on a
-separate context that is then discarded.
+runs first; its final test RMSE x 1.005 is the target.  Two BF16-stored paths
+(FP32 arithmetic) then run the same schedule, each on a fresh context: "bf16"
+rounds every stored value to the nearest bf16, "bf16_sr" rounds
+stochastically; for each the training time at which it first reaches the
+target is reported.  `value` is the "bf16" path's updates/s.  Warmup segments
+run on a separate context that is then discarded.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--segment S] [--scale F]
 
@@ -79,7 +80,9 @@ def interp_hit(rmse0, times, curve, target):
     return None
 
 
-def run_path(lib, tr, te, W0, H0, *, storage, rounding, k, hyp, nworkers, warmup, steps, segment, target=None):
+def run_path(lib, tr, te, W0, H0, *, storage, rounding, k, hyp, nworkers, warmup, steps, segment, target=None,
+             log_name=None):
+    name = log_name or storage
     if warmup > 0:
         ctx = make_ctx(lib, tr, te, W0, H0, storage=storage, rounding=rounding, k=k, hyp=hyp, nworkers=nworkers)
         for _ in range(warmup):
@@ -107,7 +110,7 @@ def run_path(lib, tr, te, W0, H0, *, storage, rounding, k, hyp, nworkers, warmup
         times.append(round(dev_s, 4))
         if target is not None and t_hit is None and rmse <= target:
             t_hit = dev_s
-        log(f"{storage}: segment {step + 1} updates={st.updates} dev={st.train_seconds * 1e3:.1f} ms "
+        log(f"{name}: segment {step + 1} updates={st.updates} dev={st.train_seconds * 1e3:.1f} ms "
             f"epochs(min over columns)={st.epochs_done} test RMSE={rmse:.6f}")
     nw = ctx.nworkers
     ctx.close()
@@ -385,7 +388,7 @@ def main() -> int:
     ap.add_argument("--nworkers", type=int, default=2048)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--only", choices=["fp32", "bf16"], default=None)
+    ap.add_argument("--only", choices=["fp32", "bf16", "bf16_sr"], default=None)
     ap.add_argument("--workload", choices=["netflix", "sweep", "ccd", "objective", "yahoo"], default="netflix")
     ap.add_argument("--epochs", type=int, default=3, help="yahoo: epochs per path")
     ap.add_argument("--ccd-rank", type=int, default=100)
@@ -429,12 +432,15 @@ def main() -> int:
         res["fp32"]["t_hit"] = next((t for t, r in zip(res["fp32"]["times"], res["fp32"]["curve"]) if r <= target),
                                     None)
     if a.only in (None, "bf16"):
-        res["bf16"] = run_path(lib, tr, te, W0, H0, storage="bf16", rounding="stochastic", target=target, **common)
+        res["bf16"] = run_path(lib, tr, te, W0, H0, storage="bf16", rounding="nearest", target=target, **common)
+    if a.only in (None, "bf16_sr"):
+        res["bf16_sr"] = run_path(lib, tr, te, W0, H0, storage="bf16", rounding="stochastic", target=target,
+                                  log_name="bf16_sr", **common)
 
     for r in res.values():
         r["t_hit_interp"] = interp_hit(r["rmse0"], r["times"], r["curve"], target)
 
-    head_name = "bf16" if "bf16" in res else "fp32"
+    head_name = next(n for n in ("bf16", "bf16_sr", "fp32") if n in res)
     head = res[head_name]
     value = head["updates"] / head["dev_s"]
     achieved = head["alg_bytes"] / head["dev_s"] / 1e9
@@ -456,7 +462,8 @@ def main() -> int:
         "config": {
             "workload": f"netflix-shape synthetic, async NOMAD token kernel, {a.segment} s training segments",
             "m": tr.m, "n": tr.n, "nnz_train": tr.nnz, "nnz_test": te.nnz, "k": k, **hyp,
-            "storage": head_name, "rounding": "stochastic" if head_name == "bf16" else "nearest",
+            "storage": "fp32" if head_name == "fp32" else "bf16",
+            "rounding": "stochastic" if head_name == "bf16_sr" else "nearest",
             "workers": head["nworkers"], "row_blocks": "balanced by ratings", "scale": a.scale,
         },
         "roofline": {
@@ -464,7 +471,8 @@ def main() -> int:
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per update (copy-calibrated), "
                               "profiles/sgd_async_kernel_netflix_k100_pmc.md, x updates per launch",
-            "kernel": f"sgd_async_kernel<{head_name}>",
+            "kernel": "sgd_async_kernel<" + {"fp32": "fp32", "bf16": "bf16, nearest",
+                                             "bf16_sr": "bf16, stochastic"}[head_name] + ">",
             "alg_bytes_per_launch": head["alg_bytes"] / a.steps,
             "avg_launch_ms": head["dev_s"] / a.steps * 1e3,
         },

@@ -19,7 +19,9 @@
 //                    fused update and 64-lane tree; pinned against ARITH_REF
 //                    (agreement to rounding) and closed forms.
 //   orc_sgd_async    nb_main.cpp:415-578; pinned by invariants (update count =
-//                    nnz * epochs, T = 1 equals the serial replay bit-for-bit).
+//                    nnz * epochs, T = 1 equals the serial replay bit-for-bit;
+//                    bf16 at T = 1 equals a float32 serial replay rounded by
+//                    torch's bf16 conversion, bit-for-bit).
 //   orc_sse          nb_main.cpp:601-667; pinned by numpy FP64.
 //   orc_loss_grad    sgd-mpi-omp.cpp:122-164 objective; pinned by the
 //                    reference DSGD binary's printed objective (oracle/_ref)
@@ -438,6 +440,14 @@ int async_run(int T, int m, int n, int k, const int64_t* colptr, const int32_t* 
               double* W, double* H, uint64_t seed, double lrate, double drate, double reg, double seconds,
               int64_t epochs, orc_async_stats* st, int storage) {
     const int per = m / T + ((m % T) > 0 ? 1 : 0);
+    // bf16 storage (A = float): every stored factor value is rounded to the
+    // nearest bf16 -- the initial factors, a row after each of its updates and
+    // a column at the end of each visit; arithmetic stays fp32
+    const bool bf16 = storage == ORC_BF16;
+    auto round_store = [&](A* v, int len) {
+        if (bf16)
+            for (int d = 0; d < len; ++d) v[d] = A(bf16_to_f(bf16_rne(float(v[d]))));
+    };
     // per-thread CSC with local rows (nb_data.cpp:131-169)
     struct Part {
         int min_row = 0, rows = 0;
@@ -456,6 +466,7 @@ int async_run(int T, int m, int n, int k, const int64_t* colptr, const int32_t* 
         P.off.assign(size_t(n) + 1, 0);
         P.w.resize(size_t(P.rows) * k);
         for (int64_t e = 0; e < int64_t(P.rows) * k; ++e) P.w[e] = A(W[int64_t(P.min_row) * k + e]);
+        for (int r = 0; r < P.rows; ++r) round_store(&P.w[int64_t(r) * k], k);
         P.counts.assign(n, 0);
     }
     // one pass: count, then place (rows stay ascending inside a column)
@@ -491,6 +502,7 @@ int async_run(int T, int m, int n, int k, const int64_t* colptr, const int32_t* 
         for (int i = 0; i < T; ++i) tk.perm[i] = i;
         std::shuffle(tk.perm.begin(), tk.perm.end(), rng_temp);
         for (int d = 0; d < k; ++d) hv[j][d] = A(H[int64_t(j) * k + d]);
+        round_store(hv[j].data(), k);
         queues[tk.perm[tk.pos]].push(&tk);
     }
     const A regA = storage == ORC_FP64 ? A(reg) : A(float(reg));
@@ -510,8 +522,10 @@ int async_run(int T, int m, int n, int k, const int64_t* colptr, const int32_t* 
                 for (int64_t o = P.off[j]; o < P.off[j + 1]; ++o) {
                     A* row = &P.w[int64_t(P.ri[o]) * k];
                     ref_update(row, col, k, P.rv[o], step, regA);
+                    round_store(row, k);
                     ++nupd;
                 }
+                round_store(col, k);
                 P.counts[j]++;
                 tk->pos++;
                 if (tk->pos >= T) {
@@ -569,7 +583,7 @@ int orc_sgd_async(int storage, int T, int m, int n, int k, const int64_t* colptr
                   const double* val, double* W, double* H, uint64_t seed, double lrate, double drate,
                   double reg, double seconds, int64_t epochs, orc_async_stats* stats) {
     if (T <= 0 || m <= 0 || n <= 0 || k <= 0 || (!(seconds > 0) && epochs <= 0)) return -1;
-    if (storage == ORC_FP32)
+    if (storage == ORC_FP32 || storage == ORC_BF16)
         return async_run<float>(T, m, n, k, colptr, rowidx, val, W, H, seed, lrate, drate, reg, seconds, epochs,
                                 stats, storage);
     if (storage == ORC_FP64)

@@ -69,7 +69,9 @@ void orc_bf16_store_pair(float x0, float x1, int rounding, uint32_t seed, uint32
  * T threads, one row part each; column tokens in per-thread FIFO queues with
  * the reference's routing (nb_main.cpp:480-575).  Stops after every column
  * completes `epochs` passes (epochs > 0) or after `seconds` (seconds > 0).
- * Storage fp32 or fp64, reference arithmetic. */
+ * Storage fp32, fp64, or bf16 with round to nearest (fp32 arithmetic; each
+ * row rounded after its update, each column at the end of its visit);
+ * reference arithmetic. */
 typedef struct orc_async_stats {
     int64_t updates;
     int64_t failures;

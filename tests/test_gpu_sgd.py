@@ -212,7 +212,9 @@ def test_async_netflix_full_size(gpu):
     k = 100, 2048 balanced workers): every epoch performs exactly nnz
     updates, and after four epochs the test RMSE of FP32 and of BF16 with
     stochastic rounding are within 0.5 % of the oracle's FP32 async
-    restatement run for the same four epochs on the host (16 threads).
+    restatement run for the same four epochs on the host (16 threads), and
+    that of BF16 with round to nearest within 0.5 % of the oracle's BF16
+    (round to nearest) async restatement.
     (After two epochs the gap was 0.47 %: the schedules differ most early.)"""
     import os
 
@@ -222,8 +224,9 @@ def test_async_netflix_full_size(gpu):
     hyp = dict(lrate=0.012, drate=0.05, reg=0.05)
     W0, H0 = lib.init_factors(tr.m, tr.n, k, 4, 12345)
     got = {}
-    for storage, rnd in (("fp32", "nearest"), ("bf16", "stochastic")):
-        print(f"[netflix full] {storage}", flush=True)
+    for name, storage, rnd in (("fp32", "fp32", "nearest"), ("bf16_sr", "bf16", "stochastic"),
+                               ("bf16", "bf16", "nearest")):
+        print(f"[netflix full] {name}", flush=True)
         ctx = lib.Context(tr.m, tr.n, k, storage=storage, rounding=rnd, mode="async", nworkers=2048,
                           seed=12345, balance=True, **hyp)
         ctx.upload(tr, te)
@@ -233,11 +236,15 @@ def test_async_netflix_full_size(gpu):
             assert st.updates == tr.nnz and st.epochs_done == ep + 1
         ev = ctx.eval()
         assert ev.test_count == te.nnz and ev.train_count == tr.nnz
-        got[storage] = np.sqrt(ev.test_sse / ev.test_count)
+        got[name] = np.sqrt(ev.test_sse / ev.test_count)
         ctx.close()
-    Wc, Hc, _ = O.sgd_async(tr, W0, H0, storage=O.FP32, T=min(16, os.cpu_count() or 1), seed=12345, epochs=4, **hyp)
-    ref = np.sqrt(O.sse(te, Wc, Hc, storage=O.FP32) / te.nnz)
-    print(f"[netflix full] test RMSE after 4 epochs: gpu fp32 {got['fp32']:.5f} bf16 {got['bf16']:.5f} "
-          f"cpu oracle fp32 {ref:.5f}", flush=True)
-    for storage, r in got.items():
-        assert abs(r - ref) <= 0.005 * ref, (storage, r, ref)
+    T = min(16, os.cpu_count() or 1)
+    ref = {}
+    for name, ost in (("fp32", O.FP32), ("bf16", O.BF16)):
+        Wc, Hc, _ = O.sgd_async(tr, W0, H0, storage=ost, T=T, seed=12345, epochs=4, **hyp)
+        ref[name] = np.sqrt(O.sse(te, Wc, Hc, storage=O.FP32) / te.nnz)
+    print(f"[netflix full] test RMSE after 4 epochs: gpu fp32 {got['fp32']:.5f} bf16-SR {got['bf16_sr']:.5f} "
+          f"bf16-RN {got['bf16']:.5f}; cpu oracle fp32 {ref['fp32']:.5f} bf16-RN {ref['bf16']:.5f}", flush=True)
+    for name, r in got.items():
+        want = ref["bf16" if name == "bf16" else "fp32"]
+        assert abs(r - want) <= 0.005 * want, (name, r, want)

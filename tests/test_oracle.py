@@ -180,6 +180,50 @@ def test_async_one_thread_is_serial():
     assert np.array_equal(Wa, Ws) and np.array_equal(Ha, Hs)
 
 
+def _bf16_rne(x):
+    """Round float32 values to the nearest bf16 (torch's conversion, ties to even)."""
+    import torch
+
+    return torch.from_numpy(np.asarray(x, np.float32).copy()).to(torch.bfloat16).to(torch.float32).numpy()
+
+
+def test_async_bf16_one_thread_matches_float32_replay():
+    """bf16 storage with round to nearest in the async restatement at T = 1
+    equals a float32 replay of the serial order in which the initial factors,
+    each row after its update and each column after its visit are rounded by
+    torch's bf16 conversion; steps large enough that rounding changes results."""
+    tr, _ = lib.synth(lib.preset(1, 0.05))
+    k = 6
+    hyp = dict(lrate=0.05, drate=0.1, reg=0.05)
+    W0, H0 = lib.init_factors(tr.m, tr.n, k, 1, 3)
+    W = _bf16_rne(W0.astype(np.float32))
+    H = _bf16_rne(H0.astype(np.float32))
+    f = np.float32
+    reg = f(hyp["reg"])
+    for e in range(2):
+        step = f(O.step(O.BF16, hyp["lrate"], hyp["drate"], e))
+        for j in range(tr.n):
+            h = H[j].copy()
+            for o in range(tr.colptr[j], tr.colptr[j + 1]):
+                i = tr.rowidx[o]
+                w = W[i].copy()
+                err = -f(tr.val[o])
+                for d in range(k):
+                    err = f(err + f(h[d] * w[d]))
+                for d in range(k):
+                    t = w[d]
+                    w[d] = f(w[d] - f(step * f(f(err * h[d]) + f(reg * t))))
+                    h[d] = f(h[d] - f(step * f(f(err * t) + f(reg * h[d]))))
+                W[i] = _bf16_rne(w)
+            H[j] = _bf16_rne(h)
+    Wa, Ha, st = O.sgd_async(tr, W0, H0, storage=O.BF16, T=1, seed=4, epochs=2, **hyp)
+    assert st.updates == 2 * tr.nnz
+    assert np.array_equal(Wa, W.astype(np.float64)) and np.array_equal(Ha, H.astype(np.float64))
+    assert not np.any(Wa.astype(np.float32).view(np.uint32) & 0xFFFF)
+    Wf, _, _ = O.sgd_async(tr, W0, H0, storage=O.FP32, T=1, seed=4, epochs=2, **hyp)
+    assert not np.array_equal(Wf, Wa)  # the rounding is really applied
+
+
 def test_async_threads_update_count():
     tr, _ = lib.synth(lib.preset(1, 0.1))
     W0, H0 = lib.init_factors(tr.m, tr.n, 8, 4, 4)
