@@ -79,10 +79,16 @@ __device__ __forceinline__ double wave_tree_sum(double x) {
 __device__ __forceinline__ float bf16_lo(uint32_t u) { return __uint_as_float(u << 16); }
 __device__ __forceinline__ float bf16_hi(uint32_t u) { return __uint_as_float(u & 0xFFFF0000u); }
 
-__device__ __forceinline__ uint32_t f32_to_bf16_rne(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return u >> 16;
+// Round an element pair to the nearest bf16 (ties to even) and pack it, x in
+// the low half: one v_cvt_pk_bf16_f32 on gfx950.  For every finite input it
+// equals adding 0x7FFF plus the kept lsb below the cut and truncating (the
+// oracle's form; denormals are preserved in these kernels); a NaN comes out
+// as a quiet NaN.
+typedef __bf16 nomad_bf16x2 __attribute__((ext_vector_type(2)));
+typedef float nomad_f32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t bf16_pack_rne(float x, float y) {
+    const nomad_f32x2 v = {x, y};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, nomad_bf16x2));
 }
 
 // Stochastic rounding: add 16 uniform random bits below the bf16 cut, truncate.
@@ -311,15 +317,10 @@ struct Store<NOMAD_STORE_BF16> {
     template <int CP = 0>
     static __device__ __forceinline__ void store(elem* row, int lane, int npairs, pair p, uint32_t rnd,
                                                  int round_mode) {
-        uint32_t lo, hi;
-        if (round_mode == NOMAD_ROUND_STOCHASTIC) {
-            lo = f32_to_bf16_sr(p.x, rnd);
-            hi = f32_to_bf16_sr(p.y, rnd >> 16);
-        } else {
-            lo = f32_to_bf16_rne(p.x);
-            hi = f32_to_bf16_rne(p.y);
-        }
-        __builtin_amdgcn_raw_buffer_store_b32(lo | (hi << 16), row_rsrc(row, npairs * 4), lane * 4, 0, CP);
+        const uint32_t word = round_mode == NOMAD_ROUND_STOCHASTIC
+                                  ? f32_to_bf16_sr(p.x, rnd) | (f32_to_bf16_sr(p.y, rnd >> 16) << 16)
+                                  : bf16_pack_rne(p.x, p.y);
+        __builtin_amdgcn_raw_buffer_store_b32(word, row_rsrc(row, npairs * 4), lane * 4, 0, CP);
     }
     static constexpr int kPairBytes = 4;
     static __device__ __forceinline__ pair load_b(__amdgpu_buffer_rsrc_t r, int voff, int soff = 0) {
@@ -343,7 +344,7 @@ struct Store<NOMAD_STORE_BF16> {
             const uint32_t hi = __float_as_uint(p.y) + (rnd >> 16);
             return __builtin_amdgcn_perm(hi, lo, 0x07060302u);
         }
-        return f32_to_bf16_rne(p.x) | (f32_to_bf16_rne(p.y) << 16);
+        return bf16_pack_rne(p.x, p.y);
     }
     template <int CP = 0>
     static __device__ __forceinline__ void store_raw_b(__amdgpu_buffer_rsrc_t r, int voff, raw v, int soff = 0) {
@@ -351,15 +352,10 @@ struct Store<NOMAD_STORE_BF16> {
     }
     static __device__ __forceinline__ void store_b(__amdgpu_buffer_rsrc_t r, int voff, int soff, pair p,
                                                    uint32_t rnd, int round_mode) {
-        uint32_t lo, hi;
-        if (round_mode == NOMAD_ROUND_STOCHASTIC) {
-            lo = f32_to_bf16_sr(p.x, rnd);
-            hi = f32_to_bf16_sr(p.y, rnd >> 16);
-        } else {
-            lo = f32_to_bf16_rne(p.x);
-            hi = f32_to_bf16_rne(p.y);
-        }
-        __builtin_amdgcn_raw_buffer_store_b32(lo | (hi << 16), r, voff, soff, 0);
+        const uint32_t word = round_mode == NOMAD_ROUND_STOCHASTIC
+                                  ? f32_to_bf16_sr(p.x, rnd) | (f32_to_bf16_sr(p.y, rnd >> 16) << 16)
+                                  : bf16_pack_rne(p.x, p.y);
+        __builtin_amdgcn_raw_buffer_store_b32(word, r, voff, soff, 0);
     }
     static __device__ __forceinline__ pair load_dc(const elem* row, int lane, int npairs) {
         const uint32_t* src = lane < npairs ? reinterpret_cast<const uint32_t*>(row)
@@ -369,15 +365,10 @@ struct Store<NOMAD_STORE_BF16> {
     }
     static __device__ __forceinline__ void store_dc(elem* row, int lane, int npairs, pair p, uint32_t rnd,
                                                     int round_mode) {
-        uint32_t lo, hi;
-        if (round_mode == NOMAD_ROUND_STOCHASTIC) {
-            lo = f32_to_bf16_sr(p.x, rnd);
-            hi = f32_to_bf16_sr(p.y, rnd >> 16);
-        } else {
-            lo = f32_to_bf16_rne(p.x);
-            hi = f32_to_bf16_rne(p.y);
-        }
-        if (lane < npairs) st_dc(reinterpret_cast<uint32_t*>(row) + lane, lo | (hi << 16));
+        const uint32_t word = round_mode == NOMAD_ROUND_STOCHASTIC
+                                  ? f32_to_bf16_sr(p.x, rnd) | (f32_to_bf16_sr(p.y, rnd >> 16) << 16)
+                                  : bf16_pack_rne(p.x, p.y);
+        if (lane < npairs) st_dc(reinterpret_cast<uint32_t*>(row) + lane, word);
     }
 };
 
