@@ -706,7 +706,18 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
             Aa tstp = Aa(0);
             uint32_t nstart = 0;
             int32_t np = 0;
+            // Everything that depends only on the column index is requested
+            // first -- its visit range and the h rows of the first four
+            // columns -- so that it shares a round trip with the token
+            // records; what those records select (next visit, step, rating
+            // window) shares the next one.
+            P hv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                hv[u] = T::load_dc(H + int64_t(readlane_i32(jj, min(u, nb - 1))) * ldf, lane, npairs);
             if (lane < nb) {
+                const int64_t vb = a.vptr[jj];
+                const int32_t nv = int32_t(a.vptr[jj + 1] - vb);
                 const uint64_t* q = reinterpret_cast<const uint64_t*>(a.tok + jj);
                 const uint64_t w0 = ld_dc(q);
                 const uint64_t w1 = ld_dc(q + 1);
@@ -715,8 +726,6 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                 tpos = int32_t(uint32_t(w1));
                 tcyc = int32_t(w1 >> 32);
                 tstp = step_for<Aa>(a, tcyc);
-                const int64_t vb = a.vptr[jj];
-                const int32_t nv = int32_t(a.vptr[jj + 1] - vb);
                 np = tpos + 1 == nv ? 0 : tpos + 1;
                 const VisitDesc nd = a.visits[vb + np];
                 nwg = nd.worker;
@@ -728,16 +737,14 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                 }
             }
             for (int t0 = 0; t0 < nb; t0 += 4) {
-                P hv[4];
                 int32_t wr[4];
                 R wv[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int t = min(t0 + u, nb - 1);
-                    const int32_t j = readlane_i32(jj, t);
                     const uint32_t st = uint32_t(readlane_i32(int32_t(tstart), t));
                     const int32_t ln = readlane_i32(tlen, t);
-                    hv[u] = T::load_dc(H + int64_t(j) * ldf, lane, npairs);
+                    if (t0 > 0) hv[u] = T::load_dc(H + int64_t(readlane_i32(jj, t)) * ldf, lane, npairs);
                     const int32_t o = min(lane, ln - 1);
                     wr[u] = rowidx[int64_t(st) + o];
                     wv[u] = val[int64_t(st) + o];
