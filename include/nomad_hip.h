@@ -72,10 +72,12 @@ typedef struct nomad_params {
     int32_t rounding;  /* NOMAD_ROUND_* (bf16 only)                              */
     int32_t nworkers;  /* async: workers = row blocks (0: 8 per compute unit);
                           strat: strata B (0: one per co-resident wave)          */
-    int32_t ptoken;    /* README --ptoken (tokens in the pipeline; the option is
-                          commented out in nomad_option.cpp:43-44).  Accepted,
-                          >= 0; the async kernel's token capacity is its LDS
-                          slots per workgroup (64, 32 for fp64), not this   */
+    int32_t ptoken;    /* README --ptoken (README.md:90, "number of tokens in the
+                          pipeline"; commented out in nomad_option.cpp:43-44):
+                          async mode holds at most this many columns resident
+                          per workgroup (0: no cap).  The workgroup's LDS
+                          slots bound it: 128 for fp32 and bf16 at 8 waves per
+                          workgroup, 64 at 4 or 16 waves, 32 for fp64       */
     int32_t device;    /* HIP device ordinal                                     */
     int32_t flags;     /* NOMAD_FLAG_* bits; others must be 0                    */
     double lrate;      /* --lrate  */
@@ -228,6 +230,28 @@ int nomad_get_nworkers(nomad_ctx* ctx, int32_t* nworkers);
  * receives the same 16 counters for each worker (wave) in turn, as far as n
  * allows.  n is the capacity of out. */
 int nomad_get_counters(nomad_ctx* ctx, int64_t* out, int32_t n);
+
+/* Route state of an async context.  The step size of a visit follows from the
+ * number of earlier visits of the column to the same worker, a count the
+ * reference keeps per (thread, column) (col_update_counts,
+ * Code/nomad/nb_main.cpp:458-459, 513-515, 544); here the count is kept per
+ * (column, workgroup) pair, since a workgroup's waves see a column in one
+ * hop.  For column j, entries vptr[j] .. vptr[j+1]-1 list the workgroups
+ * holding ratings of j in ascending order: unit[e] the workgroup, nratings[e]
+ * the column's ratings in its rows, count[e] the completed visits of j to
+ * it, route_pos[e] its position in the column's
+ * current route (the reference's perm_, Code/nomad/pool.h:11-37).
+ * tok_pos[j] is the route position of column j's next hop and tok_cycle[j]
+ * its completed route cycles (pos_ and the epoch count of the reference's
+ * ColumnData).  Any output may be NULL; *nentries (may be NULL) receives the
+ * number of pairs V.  vptr has n+1 entries, unit/count/route_pos V,
+ * tok_pos/tok_cycle n.  Builds the routes if no launch has yet. */
+int nomad_get_route_state(nomad_ctx* ctx, int64_t* vptr, int32_t* unit, int32_t* nratings, int32_t* count,
+                          int32_t* route_pos, int32_t* tok_pos, int32_t* tok_cycle, int64_t* nentries);
+
+/* Restores the visit counters of an async context (a checkpoint's counts,
+ * in the entry order of nomad_get_route_state); nentries must equal V. */
+int nomad_set_visit_counts(nomad_ctx* ctx, const int32_t* count, int64_t nentries);
 
 /* ---- CCD++ rank-one solver (the reference's competitor program) ----------
  * Replaces ccdr1_mpi at one process (Code/competitor/ccdr1-mpi-omp.cpp:398-558:

@@ -26,8 +26,18 @@
 //   * h goes back to the slot and the slot to the wave owning the column's
 //     next row; after the last piece h_j returns to HBM and the column is
 //     pushed to the next workgroup of its route; after the last workgroup of
-//     the route the column's cycle count (the step-size counter) advances,
-//     and a column whose count reaches target_cycle is parked.
+//     the route the column's cycle count advances, and a column whose count
+//     reaches target_cycle is parked.
+//
+// Step size: the reference keeps one visit count per (thread, column)
+// (nb_main.cpp:458, 514-515, 544).  Here a workgroup's waves see a column in
+// one hop, so the count is kept per (column, workgroup) pair in HBM: read when
+// the hop is claimed (the step follows from it), written back one higher when
+// the column leaves the workgroup.  At the deadline of a timed segment every
+// workgroup stops claiming and finishes the hops it holds, so every visit is
+// whole and counted, as a reference thread finishes its column before it sees
+// the stop flag (nb_main.cpp:472-570); the host then starts every column at
+// the head of a fresh route, as set_perm does (pool.cpp:32-38).
 //
 // Cross-workgroup data (h_j, the column's TokInfo) moves with device-coherent
 // (sc1) loads and stores; a producer waits for all its memory operations to
@@ -38,6 +48,7 @@
 #include "sgd_common.hpp"
 
 using namespace nomad_dev;
+using nomad_k::RouteDesc;
 using nomad_k::TokInfo;
 using nomad_k::async_slots;
 
@@ -83,11 +94,14 @@ struct AsyncShared {
     int32_t wb[SLOTS];           // the window holds range offsets [wb, we)
     int32_t we[SLOTS];
     int32_t cyc[SLOTS];          // completed route cycles of the column
-    A stp[SLOTS];                //   and the step size they give
-    int32_t npos[SLOTS];         // route position of the column's next visit
+    int32_t cs[SLOTS];           // visit counter of the (column, workgroup) pair
+    int32_t vc[SLOTS];           //   its value at the claim
+    A stp[SLOTS];                //   and the step size it gives
+    int32_t npos[SLOTS];         // route position of the column's next hop
     int32_t nwg[SLOTS];          //   its workgroup
     uint32_t nstart[SLOTS];      //   its CSC range
     int32_t nlen[SLOTS];
+    int32_t ncs[SLOTS];          //   its visit counter
     uint8_t pb[SLOTS][kPbw];     // pb[s][w]: window entries with a row below wave w's first row
     uint32_t inbox[NW][SLOTS];   // per wave: (lap << 16) | slot
     uint32_t tpush[SLOTS];       // wall clock (low 32 bits) when the slot was last handed over
@@ -95,6 +109,7 @@ struct AsyncShared {
     uint32_t freebits[SLOTS / 32];
     uint32_t lock;               // serialises ring claims and slot allocation
     uint32_t ghead;              // consumer position of this workgroup's ring
+    uint32_t stop;               // set at the deadline: no more claims
 };
 
 __device__ __forceinline__ uint32_t lds_ld(const uint32_t* p) {
@@ -144,10 +159,12 @@ __device__ __forceinline__ void write_bounds(L& sh, int s, int32_t myrow, int32_
     if (lane <= NW) sh.pb[s][lane] = uint8_t(mine);
 }
 
-__device__ __forceinline__ void tok_store(TokInfo* p, uint32_t start, int32_t len, int32_t pos, int32_t cycle) {
+__device__ __forceinline__ void tok_store(TokInfo* p, uint32_t start, int32_t len, int32_t pos, int32_t cycle,
+                                          int32_t cslot) {
     uint64_t* q = reinterpret_cast<uint64_t*>(p);
     st_dc(q, (uint64_t(uint32_t(len)) << 32) | start);
     st_dc(q + 1, (uint64_t(uint32_t(cycle)) << 32) | uint32_t(pos));
+    st_dc(q + 2, uint64_t(uint32_t(cslot)));
 }
 
 }  // namespace
@@ -185,6 +202,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
     // lane t < NW - 1 holds the first row of wave t + 1 of this workgroup
     const int32_t lane_bound = lane < NW - 1 ? wb_g[lane + 1] : INT_MAX;
     const uint32_t gmask = (1u << a.qlog2) - 1u;
+    const int slot_cap = a.slot_cap > 0 && a.slot_cap < SLOTS ? a.slot_cap : SLOTS;
     const uint64_t* ring = a.qslot + (int64_t(g) << a.qlog2);
     E* __restrict__ W = static_cast<E*>(a.W);
     E* __restrict__ H = static_cast<E*>(a.H);
@@ -209,13 +227,15 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
     if (threadIdx.x < SLOTS / 32) sh.freebits[threadIdx.x] = 0xFFFFFFFFu;
     if (threadIdx.x == 0) {
         sh.lock = 0u;
+        sh.stop = 0u;
         sh.ghead = a.qhead[int64_t(g) * nomad_k::kQStride];
     }
     __syncthreads();
 
     // Per-phase wall-clock counters only when asked for: each clock read is a
     // scalar memory round trip.  The deadline is checked every 64 streams or
-    // visits and every 8 idle polls.
+    // visits and every 8 idle polls; the first wave past it stops the
+    // workgroup's claims.
     const bool timing = a.timing != 0;
     auto clk = [timing]() -> uint64_t { return timing ? wall_clock64() : 0; };
     const uint64_t t_end = wall_clock64() + a.deadline;
@@ -225,6 +245,12 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
     int64_t tk_transit_l = 0, tk_inbox_l = 0;  // per lane
     bool hit_deadline = false;
     int idle = 0;
+    auto past_deadline = [&]() {
+        if (!hit_deadline && wall_clock64() > t_end) {
+            hit_deadline = true;
+            if (lane == 0) __hip_atomic_store(&sh.stop, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    };
 
     for (;;) {
         // ---- slots waiting in this wave's inbox -----------------------------
@@ -421,6 +447,8 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                         t_np = sh.npos[ts];
                         t_ncyc = t_cyc + (t_np == 0 ? 1 : 0);
                         t_park = t_np == 0 && t_ncyc >= a.target_cycle;
+                        // the pair's visit is complete: its counter goes up by one
+                        st_dc(a.vcnt + sh.cs[ts], sh.vc[ts] + 1);
                     }
                     for (uint64_t mm = xmask; mm != 0; mm &= mm - 1) {
                         const int t = int(__builtin_ctzll(mm));
@@ -434,7 +462,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                         T::store_dc(H + int64_t(j) * ldf, lane, npairs, hx, rnd, RM);
                         if (lane == 0) {
                             tok_store(a.tok + j, sh.nstart[s], sh.nlen[s], readlane_i32(t_np, t),
-                                      readlane_i32(t_ncyc, t));
+                                      readlane_i32(t_ncyc, t), sh.ncs[s]);
                             if (timing) st_dc(a.tok_stamp + j, uint64_t(wall_clock64()));
                         }
                     }
@@ -461,10 +489,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                     n_exit += __popcll(xmask);
                     tk_exit += int64_t(clk() - t_v1);
                 }
-                if ((n_work & 63) == 0 && wall_clock64() > t_end) {
-                    hit_deadline = true;
-                    break;
-                }
+                if ((n_work & 63) == 0) past_deadline();
                 continue;
             }
 
@@ -612,7 +637,10 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                 const bool park = np == 0 && ncyc >= a.target_cycle;
                 const uint32_t rnd = sr_bits_k(a.sr_seed, 0x80000000u | uint32_t(g), sr_b, lkey);
                 T::store_dc(H + int64_t(j) * ldf, lane, npairs, h, rnd, RM);
-                if (lane == 0) tok_store(a.tok + j, sh.nstart[s], sh.nlen[s], np, ncyc);
+                if (lane == 0) {
+                    tok_store(a.tok + j, sh.nstart[s], sh.nlen[s], np, ncyc, sh.ncs[s]);
+                    st_dc(a.vcnt + sh.cs[s], sh.vc[s] + 1);  // the pair's visit is complete
+                }
                 if (park) {
                     wait_all_mem();
                     if (lane == 0) __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -636,10 +664,27 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                 ++n_exit;
                 tk_exit += int64_t(clk() - t_v1);
             }
-            if ((n_work & 63) == 0 && wall_clock64() > t_end) {
-                hit_deadline = true;
-                break;
+            if ((n_work & 63) == 0) past_deadline();
+            continue;
+        }
+
+        // ---- past the deadline: no claims; leave once every slot is free -----
+        // (taken under the claim lock, so no claim that started before the
+        // stop can allocate a slot after this wave found them all free)
+        if (__builtin_amdgcn_readfirstlane(lds_ld(&sh.stop)) != 0u) {
+            uint32_t lk = 1u, used = 1u;
+            if (lane == 0) {
+                lk = __hip_atomic_exchange(&sh.lock, 1u, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (lk == 0u) {
+                    used = 0u;
+#pragma unroll
+                    for (int w = 0; w < SLOTS / 32; ++w) used |= ~lds_ld(&sh.freebits[w]);
+                    __hip_atomic_store(&sh.lock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
             }
+            if (__builtin_amdgcn_readfirstlane(used) == 0u) break;
+            ++n_idle;
+            __builtin_amdgcn_s_sleep(2);
             continue;
         }
 
@@ -650,6 +695,12 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
         int nb = 0;
         int32_t myslot = 0;
         uint64_t sv = 0;
+        // (the stop flag is read again under the lock: a wave that stops the
+        // workgroup takes the lock before it checks for free slots)
+        if (lk == 0u && __builtin_amdgcn_readfirstlane(lds_ld(&sh.stop)) != 0u) {
+            if (lane == 0) __hip_atomic_store(&sh.lock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            lk = 1u;
+        }
         if (lk == 0u) {
             // Shared LDS words are read with atomic loads and every value that
             // selects a slot is made wave-uniform: lane 0 allocates, the wave
@@ -671,7 +722,9 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
 #pragma unroll
             for (int w = 0; w < SLOTS / 32; ++w) nfree += __popc(lds_ld(&sh.freebits[w]));
             nfree = __builtin_amdgcn_readfirstlane(nfree);
-            nb = min(nb, min(nfree, kFetchBatch));
+            // at most slot_cap columns resident (the --ptoken cap)
+            nb = min(nb, min(nfree - (SLOTS - slot_cap), kFetchBatch));
+            nb = max(nb, 0);
             ++n_fetch;
             n_res += SLOTS - nfree;
             for (int t = 0; t < nb; ++t) {
@@ -702,15 +755,16 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
             // lane t < nb: metadata of claimed column t
             const int32_t jj = int32_t(uint32_t(sv));
             uint32_t tstart = 0;
-            int32_t tlen = 1, tpos = 0, tcyc = 0, nwg = 0, nlen = 1;
+            int32_t tlen = 1, tpos = 0, tcyc = 0, nwg = 0, nlen = 1, tcs = 0, tvc = 0, ncs = 0;
             Aa tstp = Aa(0);
             uint32_t nstart = 0;
             int32_t np = 0;
             // Everything that depends only on the column index is requested
-            // first -- its visit range and the h rows of the first four
+            // first -- its route range and the h rows of the first four
             // columns -- so that it shares a round trip with the token
-            // records; what those records select (next visit, step, rating
-            // window) shares the next one.
+            // records; what those records select (next hop, visit count,
+            // rating window) shares the next one, and the step is computed
+            // from the count after the window loads are issued.
             P hv[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u)
@@ -721,16 +775,19 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                 const uint64_t* q = reinterpret_cast<const uint64_t*>(a.tok + jj);
                 const uint64_t w0 = ld_dc(q);
                 const uint64_t w1 = ld_dc(q + 1);
+                const uint64_t w2 = ld_dc(q + 2);
                 tstart = uint32_t(w0);
                 tlen = int32_t(w0 >> 32);
                 tpos = int32_t(uint32_t(w1));
                 tcyc = int32_t(w1 >> 32);
-                tstp = step_for<Aa>(a, tcyc);
+                tcs = int32_t(uint32_t(w2));
                 np = tpos + 1 == nv ? 0 : tpos + 1;
-                const VisitDesc nd = a.visits[vb + np];
-                nwg = nd.worker;
-                nstart = uint32_t(nd.start);
+                const RouteDesc nd = a.route[vb + np];
+                nwg = nd.unit;
+                nstart = nd.start;
                 nlen = nd.len;
+                ncs = nd.cslot;
+                tvc = ld_dc(a.vcnt + tcs);
                 if (timing) {
                     const uint64_t stamp = ld_dc(a.tok_stamp + jj);
                     if (stamp != 0) tk_transit_l += int64_t(wall_clock64() - stamp);
@@ -749,6 +806,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                     wr[u] = rowidx[int64_t(st) + o];
                     wv[u] = val[int64_t(st) + o];
                 }
+                if (t0 == 0) tstp = lane < nb ? Aa(step_of_count(a.lrate, a.drate, tvc)) : Aa(0);
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int t = t0 + u;
@@ -768,11 +826,14 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                             sh.wb[s] = 0;
                             sh.we[s] = min(kWave, ln);
                             sh.cyc[s] = readlane_i32(tcyc, t);
+                            sh.cs[s] = readlane_i32(tcs, t);
+                            sh.vc[s] = readlane_i32(tvc, t);
                             sh.stp[s] = readlane_val(tstp, t);
                             sh.npos[s] = readlane_i32(np, t);
                             sh.nwg[s] = readlane_i32(nwg, t);
                             sh.nstart[s] = uint32_t(readlane_i32(int32_t(nstart), t));
                             sh.nlen[s] = readlane_i32(nlen, t);
+                            sh.ncs[s] = readlane_i32(ncs, t);
                         }
                         inbox_push<SLOTS>(sh, wave_of(lane_bound, readlane_i32(wr[u], 0)), s, timing);
                     }
@@ -785,10 +846,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
         // the shared completion counter is read only now and then: every idle
         // wave polling one word would load a single memory channel
         if ((idle & 15) == 15 && int32_t(ld_dc(a.done)) >= a.ncols_active) break;
-        if ((idle & 7) == 7 && wall_clock64() > t_end) {
-            hit_deadline = true;
-            break;
-        }
+        if ((idle & 7) == 7) past_deadline();
         if (idle < 8)
             __builtin_amdgcn_s_sleep(2);
         else
@@ -796,7 +854,9 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
         ++idle;
     }
 
-    // ---- exit: columns still resident in LDS go back to HBM ---------------------
+    // ---- exit: every slot is free here (a deadline drains the claimed hops;
+    // otherwise every column is parked); h of any column still resident would
+    // go back to HBM ----------------------------------------------------------
     __syncthreads();
     for (int s = wave; s < SLOTS; s += NW) {
         if (lds_ld(&sh.freebits[s >> 5]) & (1u << (s & 31))) continue;

@@ -23,6 +23,7 @@
 #include "kernel_args.h"
 #include "nomad_hip.h"
 
+using nomad_k::RouteDesc;
 using nomad_k::SgdArgs;
 using nomad_k::SseArgs;
 using nomad_k::TokInfo;
@@ -272,11 +273,13 @@ struct nomad_ctx {
 
     // async routes and rings
     bool routes_ready = false;
+    bool routes_used = false;  // a launch has run on the current routes
     int64_t V = 0;
     std::vector<int64_t> h_vptr;
-    std::vector<VisitDesc> h_visits;
+    std::vector<RouteDesc> h_route;  // per column its hops in route order
     int64_t* d_vptr = nullptr;
-    VisitDesc* d_visits = nullptr;
+    RouteDesc* d_route = nullptr;
+    int32_t* d_vcnt = nullptr;       // per (column, workgroup) pair, index RouteDesc::cslot: visits so far
     TokInfo* d_tok = nullptr;
     uint64_t* d_tok_stamp = nullptr;
     uint64_t* d_qslot = nullptr;
@@ -316,7 +319,8 @@ struct nomad_ctx {
         dev_free(d_steps);
         dev_free(d_partial);
         dev_free(d_vptr);
-        dev_free(d_visits);
+        dev_free(d_route);
+        dev_free(d_vcnt);
         dev_free(d_tile_ptr);
         dev_free(d_tile_visits);
         dev_free(d_wbound);
@@ -512,7 +516,7 @@ void reperm_routes(nomad_ctx* c) {
         const int32_t j0 = int32_t(int64_t(n) * q / kChunks), j1 = int32_t(int64_t(n) * (q + 1) / kChunks);
         for (int32_t j = j0; j < j1; ++j) {
             uint64_t st = key ^ (uint64_t(j) * 0xD1B54A32D192ED03ull);
-            VisitDesc* v = c->h_visits.data() + c->h_vptr[j];
+            RouteDesc* v = c->h_route.data() + c->h_vptr[j];
             const int64_t len = c->h_vptr[j + 1] - c->h_vptr[j];
             for (int64_t i = len - 1; i > 0; --i) {
                 const int64_t r = int64_t(splitmix64(st) % uint64_t(i + 1));
@@ -535,12 +539,14 @@ void reperm_routes(nomad_ctx* c) {
 // seeded seed + 137, nb_main.cpp:79), restricted to the workgroups that hold
 // ratings of the column; each such (column, workgroup) pair is one run of
 // the CSC column (rows ascending, so a workgroup's rows are contiguous).
+// The pair's visit counter is its index in workgroup order: entry
+// vptr[j] + i counts the visits of column j's i-th workgroup (ascending).
 void build_routes(nomad_ctx* c, std::mt19937_64& rng) {
     const int32_t n = c->p.n, P = c->units;
     std::vector<int32_t> perm(P), rank(P);
-    std::vector<VisitDesc> runs;
+    std::vector<RouteDesc> runs;
     std::vector<int64_t> vptr(size_t(n) + 1, 0);
-    std::vector<VisitDesc> visits;
+    std::vector<RouteDesc> route;
     for (int32_t j = 0; j < n; ++j) {
         for (int32_t q = 0; q < P; ++q) perm[q] = q;
         std::shuffle(perm.begin(), perm.end(), rng);
@@ -551,30 +557,30 @@ void build_routes(nomad_ctx* c, std::mt19937_64& rng) {
             const int32_t wk = c->row_unit[c->h_rowidx[e]];
             int64_t f = e;
             while (f < c1 && c->row_unit[c->h_rowidx[f]] == wk) ++f;
-            runs.push_back(VisitDesc{e, int32_t(f - e), wk});
+            runs.push_back(RouteDesc{uint32_t(e), int32_t(f - e), wk, int32_t(int64_t(route.size()) + int64_t(runs.size()))});
             e = f;
         }
         std::sort(runs.begin(), runs.end(),
-                  [&](const VisitDesc& a, const VisitDesc& b) { return rank[a.worker] < rank[b.worker]; });
-        visits.insert(visits.end(), runs.begin(), runs.end());
-        vptr[j + 1] = int64_t(visits.size());
+                  [&](const RouteDesc& a, const RouteDesc& b) { return rank[a.unit] < rank[b.unit]; });
+        route.insert(route.end(), runs.begin(), runs.end());
+        vptr[j + 1] = int64_t(route.size());
     }
     c->h_vptr.swap(vptr);
-    c->h_visits.swap(visits);
-    c->V = int64_t(c->h_visits.size());
+    c->h_route.swap(route);
+    c->V = int64_t(c->h_route.size());
 }
 
-// Routes, rings and token records for the next launch.  Every call starts
-// every active column at the first workgroup of its route; from the second
-// call on the routes are re-drawn first, as the reference re-permutes every
-// column after each evaluation (nb_main.cpp:313-321).
-int setup_async(nomad_ctx* c) {
+// Row blocks, routes, rings, token records and visit counters of an async
+// context, built once per upload of the training data (before the first
+// launch, or when the host restores visit counters).
+int build_async_state(nomad_ctx* c) {
     int rc;
     const int32_t n = c->p.n, P = c->units;
     std::vector<TokInfo> tok(static_cast<size_t>(n));
-    if (!c->routes_ready) {
+    {
         c->h_vptr.clear();
-        c->h_visits.clear();
+        c->h_route.clear();
+        if (c->tr.nnz >= (int64_t(1) << 32)) return fail(NOMAD_EINVAL, "async mode: 2^32 or more ratings");
         build_partition(c);
         build_routes(c, c->rng_route);
         // ring capacity: more than the number of columns that can ever be
@@ -583,15 +589,15 @@ int setup_async(nomad_ctx* c) {
         c->ncols_active = 0;
         for (int32_t j = 0; j < n; ++j) {
             if (c->h_vptr[j + 1] > c->h_vptr[j]) ++c->ncols_active;
-            for (int64_t v = c->h_vptr[j]; v < c->h_vptr[j + 1]; ++v) per_unit[c->h_visits[v].worker]++;
+            for (int64_t v = c->h_vptr[j]; v < c->h_vptr[j + 1]; ++v) per_unit[c->h_route[v].unit]++;
         }
         const int32_t maxq = P ? *std::max_element(per_unit.begin(), per_unit.end()) : 0;
         c->qlog2 = 1;
         while ((int64_t(1) << c->qlog2) < int64_t(maxq) + 1) ++c->qlog2;
-        if (c->V >= (int64_t(1) << 31) || c->tr.nnz >= (int64_t(1) << 32))
-            return fail(NOMAD_EINVAL, "async mode: more than 2^31 visits or 2^32 ratings");
+        if (c->V >= (int64_t(1) << 31)) return fail(NOMAD_EINVAL, "async mode: 2^31 or more route hops");
         dev_free(c->d_vptr);
-        dev_free(c->d_visits);
+        dev_free(c->d_route);
+        dev_free(c->d_vcnt);
         dev_free(c->d_wbound);
         coh_free(c->d_tok);
         coh_free(c->d_tok_stamp);
@@ -602,7 +608,11 @@ int setup_async(nomad_ctx* c) {
         coh_free(c->d_timed_out);
         dev_free(c->d_stats);
         if ((rc = dev_alloc(&c->d_vptr, size_t(n) + 1))) return rc;
-        if ((rc = dev_alloc(&c->d_visits, size_t(c->V)))) return rc;
+        if ((rc = dev_alloc(&c->d_route, size_t(c->V)))) return rc;
+        // visit counters start at zero, like the reference's per-thread
+        // col_update_counts (nb_main.cpp:458-459), and persist across segments
+        if ((rc = dev_alloc(&c->d_vcnt, size_t(c->V)))) return rc;
+        NOMAD_HIP_TRY(hipMemset(c->d_vcnt, 0, size_t(c->V) * 4));
         if ((rc = dev_alloc(&c->d_wbound, c->h_wbound.size()))) return rc;
         NOMAD_HIP_TRY(hipMemcpy(c->d_wbound, c->h_wbound.data(), c->h_wbound.size() * 4, hipMemcpyHostToDevice));
         if ((rc = coh_alloc(c->p.device, &c->d_tok, size_t(n)))) return rc;
@@ -621,28 +631,51 @@ int setup_async(nomad_ctx* c) {
         NOMAD_HIP_TRY(hipMemset(c->d_qhead, 0, size_t(P) * nomad_k::kQStride * 4));
         NOMAD_HIP_TRY(hipMemset(c->d_stats, 0, nstat * 8));
         c->stats_prev.assign(nstat, 0);
-        for (auto& t : tok) t = TokInfo{0, 0, 0, 0};
-        c->routes_ready = true;
+        for (auto& t : tok) t = TokInfo{0, 0, 0, 0, 0, 0};
         c->segments = 0;
         c->cycle_base = 0;
-    } else {
-        NOMAD_HIP_TRY(hipMemcpy(tok.data(), c->d_tok, size_t(n) * sizeof(TokInfo), hipMemcpyDeviceToHost));
-        reperm_routes(c);
-        // drop whatever is still queued: head = tail on every ring
-        NOMAD_HIP_TRY(hipMemcpy(c->d_qhead, c->d_qtail, size_t(P) * nomad_k::kQStride * 4, hipMemcpyDeviceToDevice));
     }
     for (int32_t j = 0; j < n; ++j) {
         if (c->h_vptr[j + 1] > c->h_vptr[j]) {
-            const VisitDesc& v = c->h_visits[c->h_vptr[j]];
-            tok[j].start = uint32_t(v.start);
+            const RouteDesc& v = c->h_route[c->h_vptr[j]];
+            tok[j].start = v.start;
             tok[j].len = v.len;
+            tok[j].cslot = v.cslot;
+        }
+    }
+    NOMAD_HIP_TRY(hipMemcpy(c->d_vptr, c->h_vptr.data(), c->h_vptr.size() * 8, hipMemcpyHostToDevice));
+    NOMAD_HIP_TRY(hipMemcpy(c->d_route, c->h_route.data(), size_t(c->V) * sizeof(RouteDesc), hipMemcpyHostToDevice));
+    NOMAD_HIP_TRY(hipMemcpy(c->d_tok, tok.data(), size_t(n) * sizeof(TokInfo), hipMemcpyHostToDevice));
+    c->routes_ready = true;
+    c->routes_used = false;
+    return NOMAD_OK;
+}
+
+// Routes, rings and token records for the next launch.  Every launch starts
+// every active column at the first workgroup of its route; routes a launch
+// has used are re-drawn first, as the reference re-permutes every column
+// after each evaluation (nb_main.cpp:313-321, set_perm pool.cpp:32-38).
+int setup_async(nomad_ctx* c) {
+    if (!c->routes_ready) return build_async_state(c);
+    if (!c->routes_used) return NOMAD_OK;
+    const int32_t n = c->p.n, P = c->units;
+    std::vector<TokInfo> tok(static_cast<size_t>(n));
+    NOMAD_HIP_TRY(hipMemcpy(tok.data(), c->d_tok, size_t(n) * sizeof(TokInfo), hipMemcpyDeviceToHost));
+    reperm_routes(c);
+    // drop whatever is still queued: head = tail on every ring
+    NOMAD_HIP_TRY(hipMemcpy(c->d_qhead, c->d_qtail, size_t(P) * nomad_k::kQStride * 4, hipMemcpyDeviceToDevice));
+    for (int32_t j = 0; j < n; ++j) {
+        if (c->h_vptr[j + 1] > c->h_vptr[j]) {
+            const RouteDesc& v = c->h_route[c->h_vptr[j]];
+            tok[j].start = v.start;
+            tok[j].len = v.len;
+            tok[j].cslot = v.cslot;
         }
         tok[j].pos = 0;
     }
-    NOMAD_HIP_TRY(hipMemcpy(c->d_vptr, c->h_vptr.data(), c->h_vptr.size() * 8, hipMemcpyHostToDevice));
-    NOMAD_HIP_TRY(hipMemcpy(c->d_visits, c->h_visits.data(), size_t(c->V) * sizeof(VisitDesc),
-                            hipMemcpyHostToDevice));
+    NOMAD_HIP_TRY(hipMemcpy(c->d_route, c->h_route.data(), size_t(c->V) * sizeof(RouteDesc), hipMemcpyHostToDevice));
     NOMAD_HIP_TRY(hipMemcpy(c->d_tok, tok.data(), size_t(n) * sizeof(TokInfo), hipMemcpyHostToDevice));
+    c->routes_used = false;
     return NOMAD_OK;
 }
 
@@ -779,13 +812,13 @@ int train_async(nomad_ctx* c, double seconds, int64_t epochs, nomad_train_stats*
     // column order
     std::vector<int64_t> off(size_t(P) + 1, 0);
     for (int32_t j = 0; j < n; ++j)
-        if (c->h_vptr[j + 1] > c->h_vptr[j]) off[c->h_visits[c->h_vptr[j]].worker + 1]++;
+        if (c->h_vptr[j + 1] > c->h_vptr[j]) off[c->h_route[c->h_vptr[j]].unit + 1]++;
     for (int32_t q = 0; q < P; ++q) off[q + 1] += off[q];
     std::vector<int32_t> list(static_cast<size_t>(off[P]));
     {
         std::vector<int64_t> fill(off.begin(), off.end() - 1);
         for (int32_t j = 0; j < n; ++j)
-            if (c->h_vptr[j + 1] > c->h_vptr[j]) list[fill[c->h_visits[c->h_vptr[j]].worker]++] = j;
+            if (c->h_vptr[j + 1] > c->h_vptr[j]) list[fill[c->h_route[c->h_vptr[j]].unit]++] = j;
     }
     int32_t* d_list = nullptr;
     int64_t* d_off = nullptr;
@@ -804,14 +837,11 @@ int train_async(nomad_ctx* c, double seconds, int64_t epochs, nomad_train_stats*
         return fail(NOMAD_EHIP, "copy of ring seeds failed");
     }
     const int64_t target = epochs > 0 ? c->cycle_base + epochs : int64_t(INT_MAX);
-    if ((rc = ensure_steps(c, std::min<int64_t>(target, c->cycle_base + 100000) + 2))) {
-        cleanup();
-        return rc;
-    }
     SgdArgs a = base_args(c);
     a.wbound = c->d_wbound;
     a.vptr = c->d_vptr;
-    a.visits = c->d_visits;
+    a.route = c->d_route;
+    a.vcnt = c->d_vcnt;
     a.tok = c->d_tok;
     a.tok_stamp = c->d_tok_stamp;
     a.qslot = c->d_qslot;
@@ -824,6 +854,7 @@ int train_async(nomad_ctx* c, double seconds, int64_t epochs, nomad_train_stats*
     a.target_cycle = int32_t(std::min<int64_t>(target, INT_MAX));
     a.ncols_active = c->ncols_active;
     a.timing = (c->p.flags & NOMAD_FLAG_TIMING) ? 1 : 0;
+    a.slot_cap = c->p.ptoken;
     a.epoch = int32_t(c->async_launches++ & 0x7FFFFFFF);
     const double limit = seconds > 0 ? seconds : watchdog_seconds();
     a.deadline = uint64_t(limit * c->wall_hz);
@@ -859,6 +890,7 @@ int train_async(nomad_ctx* c, double seconds, int64_t epochs, nomad_train_stats*
     for (int i = 0; i < kStatWidth; ++i) c->last_stats[i] = sum[i];
     const int64_t upd = sum[nomad_k::kStatUpdates], vis = sum[nomad_k::kStatVisits], fails = sum[nomad_k::kStatIdle];
     ++c->segments;
+    c->routes_used = true;
     const bool all_parked = int32_t(done) >= c->ncols_active;
     if (all_parked && epochs > 0) {
         c->cycle_base = target;
@@ -1235,6 +1267,55 @@ int nomad_get_counters(nomad_ctx* c, int64_t* out, int32_t n) {
     for (int32_t i = 0; i < std::min<int32_t>(n, kStatWidth); ++i) out[i] = c->last_stats[i];
     for (int64_t i = kStatWidth; i < n && size_t(i - kStatWidth) < c->last_unit_stats.size(); ++i)
         out[i] = c->last_unit_stats[size_t(i - kStatWidth)];
+    return NOMAD_OK;
+}
+
+int nomad_get_route_state(nomad_ctx* c, int64_t* vptr, int32_t* unit, int32_t* nratings, int32_t* count,
+                          int32_t* route_pos, int32_t* tok_pos, int32_t* tok_cycle, int64_t* nentries) {
+    if (!c) return fail(NOMAD_EINVAL, "nomad_get_route_state: null context");
+    if (c->p.mode != NOMAD_MODE_ASYNC) return fail(NOMAD_ESTATE, "nomad_get_route_state: not an async context");
+    if (c->tr.nnz < 0) return fail(NOMAD_ESTATE, "nomad_get_route_state: no training data uploaded");
+    NOMAD_HIP_TRY(hipSetDevice(c->p.device));
+    NOMAD_HIP_TRY(hipStreamSynchronize(c->stream));
+    int rc;
+    if (!c->routes_ready && (rc = build_async_state(c))) return rc;
+    const int32_t n = c->p.n;
+    if (nentries) *nentries = c->V;
+    if (vptr) std::copy(c->h_vptr.begin(), c->h_vptr.end(), vptr);
+    // entries in workgroup order: entry cslot of the route's hop with that index
+    if (unit || nratings || route_pos) {
+        for (int32_t j = 0; j < n; ++j)
+            for (int64_t e = c->h_vptr[j]; e < c->h_vptr[j + 1]; ++e) {
+                const RouteDesc& r = c->h_route[size_t(e)];
+                if (unit) unit[r.cslot] = r.unit;
+                if (nratings) nratings[r.cslot] = r.len;
+                if (route_pos) route_pos[r.cslot] = int32_t(e - c->h_vptr[j]);
+            }
+    }
+    if (count) NOMAD_HIP_TRY(hipMemcpy(count, c->d_vcnt, size_t(c->V) * 4, hipMemcpyDeviceToHost));
+    if (tok_pos || tok_cycle) {
+        std::vector<TokInfo> tok(static_cast<size_t>(n));
+        NOMAD_HIP_TRY(hipMemcpy(tok.data(), c->d_tok, size_t(n) * sizeof(TokInfo), hipMemcpyDeviceToHost));
+        for (int32_t j = 0; j < n; ++j) {
+            if (tok_pos) tok_pos[j] = tok[j].pos;
+            if (tok_cycle) tok_cycle[j] = tok[j].cycle;
+        }
+    }
+    return NOMAD_OK;
+}
+
+int nomad_set_visit_counts(nomad_ctx* c, const int32_t* count, int64_t nentries) {
+    if (!c || !count) return fail(NOMAD_EINVAL, "nomad_set_visit_counts: null argument");
+    if (c->p.mode != NOMAD_MODE_ASYNC) return fail(NOMAD_ESTATE, "nomad_set_visit_counts: not an async context");
+    if (c->tr.nnz < 0) return fail(NOMAD_ESTATE, "nomad_set_visit_counts: no training data uploaded");
+    NOMAD_HIP_TRY(hipSetDevice(c->p.device));
+    NOMAD_HIP_TRY(hipStreamSynchronize(c->stream));
+    int rc;
+    if (!c->routes_ready && (rc = build_async_state(c))) return rc;
+    if (nentries != c->V) return fail(NOMAD_EINVAL, "nomad_set_visit_counts: nentries != the context's pairs");
+    for (int64_t e = 0; e < nentries; ++e)
+        if (count[e] < 0) return fail(NOMAD_EINVAL, "nomad_set_visit_counts: negative count");
+    NOMAD_HIP_TRY(hipMemcpy(c->d_vcnt, count, size_t(nentries) * 4, hipMemcpyHostToDevice));
     return NOMAD_OK;
 }
 

@@ -14,15 +14,30 @@ struct VisitDesc {
     int32_t worker;
 };
 
+// One hop of an async column route: the column's ratings in workgroup
+// `unit`'s rows are the CSC range [start, start + len); `cslot` indexes the
+// (column, workgroup) pair's visit counter.  A column's hops are listed in
+// route order; the counter index stays with the pair when the host
+// re-permutes the route.
+struct RouteDesc {
+    uint32_t start;
+    int32_t len;
+    int32_t unit;
+    int32_t cslot;
+};
+
 // Per-column token state kept in HBM between holders: the CSC range of the
-// column's next visit, its position in the column's route and the number of
-// completed route cycles (the step-size counter).  Written with
-// device-coherent stores before the column is published on a ring.
-struct alignas(16) TokInfo {
+// column's next hop, its position in the column's route, the number of
+// completed route cycles (for parking after a number of epochs) and the
+// hop's visit-counter index.  Written with device-coherent stores before the
+// column is published on a ring.
+struct alignas(8) TokInfo {
     uint32_t start;
     int32_t len;
     int32_t pos;
     int32_t cycle;
+    int32_t cslot;
+    int32_t pad;
 };
 
 // uint32 words between consecutive workgroups' ring counters (qtail, qhead):
@@ -64,9 +79,10 @@ struct SgdArgs {
 
     // async schedule (two-level: workgroups of waves)
     const int32_t* wbound;     // nblocks * waves + 1 first rows of the worker row blocks
-    const int64_t* vptr;       // n + 1 offsets into visits
-    const VisitDesc* visits;   // per column, in route order; .worker = workgroup
-    TokInfo* tok;              // per column: its next visit and completed cycles
+    const int64_t* vptr;       // n + 1 offsets into route
+    const RouteDesc* route;    // per column, its hops in route order
+    int32_t* vcnt;             // per (column, workgroup) pair: visits so far (the step-size counter)
+    TokInfo* tok;              // per column: its next hop and completed cycles
     uint64_t* tok_stamp;       // per column: wall clock when it was last published (diagnostics)
     uint64_t* qslot;           // nblocks x (1 << qlog2) ring slots: (lap+1) << 32 | column
     uint32_t* qtail;           // per workgroup (stride kQStride), producers' reservation counter
@@ -78,9 +94,11 @@ struct SgdArgs {
     int32_t target_cycle;      // park a column when its cycle count reaches this
     int32_t ncols_active;      // columns circulating in this launch
     int32_t timing;            // nonzero: collect the per-phase wall-clock counters
+    int32_t slot_cap;          // > 0: at most this many columns resident per workgroup
     uint64_t deadline;         // wall_clock64 ticks after its start at which a worker stops
 
     // strat schedule
+    const VisitDesc* visits;   // strat tile lists; .worker = column
     const int64_t* tile_ptr;   // B * B + 1: first visit of tile (b, cb) at [b * B + cb] in visits
     int32_t substep;
     int32_t epoch;             // strat: the epoch; async: the launch index

@@ -132,7 +132,9 @@ bool parse_args(int argc, char** argv, Options& o) {
         {"r0delay", 0, "accepted, no effect at one rank", Spec::kIgnoredValue, &ignored_d},
         {"input", 'i', "path of data (directory with train.dat and test.dat)", Spec::kString, &o.input},
         {"path", 0, "the same as --input (README spelling)", Spec::kString, &o.input},
-        {"ptoken", 0, "README's tokens in the pipeline: accepted, >= 0 (the GPU holds 64 per workgroup)",
+        {"ptoken", 0,
+         "tokens in the pipeline: at most this many columns resident per GPU workgroup (0: the LDS slots, 128 "
+         "for fp32/bf16 at 8 waves per workgroup, 64 at 4 or 16, 32 for fp64)",
          Spec::kInt, &o.ptoken},
         {"output", 'o', "path prefix of the factor dump (<output>0)", Spec::kString, &o.output},
         {"recover", 0, "accepted, no effect", Spec::kIgnoredValue, &ignored_d},

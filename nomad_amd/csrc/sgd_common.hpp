@@ -22,6 +22,25 @@ __device__ __forceinline__ A step_for(const SgdArgs& a, int32_t count) {
     return A(s);
 }
 
+// Step size of visit count c (nb_main.cpp:514-515) computed on the device, so
+// that a count read from memory needs no further table lookup:
+// lrate * 1.5 / (1 + drate * (c + 1)^1.5) in double.  (c + 1)^1.5 is x * sqrt(x)
+// with the rounding errors of the square root and of the product added back
+// (both are exact fma residuals), which gives the correctly rounded power.
+// Rounded to float it equals the host's step table (std::pow) for every count
+// below 2e7 at the learning and decay rates tried; in double it differs from
+// glibc's pow, which is not always correctly rounded, in the last bit of
+// about one count in a thousand.
+__device__ __forceinline__ double step_of_count(double lrate, double drate, int32_t c) {
+    const double x = double(c) + 1.0;
+    const double s = __builtin_sqrt(x);
+    const double r = __builtin_fma(-s, s, x);  // x - s^2, exact
+    const double p = x * s;
+    const double e = __builtin_fma(x, s, -p);  // x s - p, exact
+    const double x15 = p + (e + x * (r / (2.0 * s)));
+    return lrate * 1.5 / (1.0 + drate * x15);
+}
+
 // Row access of W / H for the stratified kernels: plain (DC = false) when
 // every reader of a row runs after a kernel boundary, device-coherent (the
 // sc1 cache policy of relaxed agent-scope atomics, DC = true) when a row

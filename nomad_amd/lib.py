@@ -112,6 +112,8 @@ HIP_SYMBOLS = {
     "nomad_get_nworkers": (C.c_int, [_P, _i32p]),
     "nomad_row_blocks": (C.c_int, [_i32p, C.c_int64, C.c_int32, C.c_int32, _f64p, _i32p]),
     "nomad_get_counters": (C.c_int, [_P, _i64p, C.c_int32]),
+    "nomad_get_route_state": (C.c_int, [_P, _i64p, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _i64p]),
+    "nomad_set_visit_counts": (C.c_int, [_P, _i32p, C.c_int64]),
     "nomad_train_batch": (C.c_int, [C.POINTER(_P), C.c_int32, C.c_int64, C.c_int32, C.POINTER(TrainStats)]),
     "nomad_autotune": (C.c_int, [C.POINTER(Params), _i64p, _i32p, _f64p, C.c_int64, C.c_double, C.POINTER(TuneTrial),
                                  C.POINTER(TuneTrial), C.c_int32, _i32p]),
@@ -438,6 +440,27 @@ class Context:
         self._check(self._lib.nomad_get_counters(self._h, _ptr(out, C.c_int64), out.size))
         return out[16:].reshape(units, 16)
 
+    def route_state(self) -> dict:
+        """Route state of an async context (nomad_get_route_state): per
+        (column, workgroup) pair in column-then-workgroup order its workgroup,
+        ratings, visit count and position in the current route; per column
+        the position of its next hop and its completed route cycles."""
+        V = C.c_int64()
+        self._check(self._lib.nomad_get_route_state(self._h, None, None, None, None, None, None, None, C.byref(V)))
+        V = V.value
+        out = dict(vptr=np.zeros(self.n + 1, np.int64), unit=np.zeros(V, np.int32),
+                   nratings=np.zeros(V, np.int32), count=np.zeros(V, np.int32), route_pos=np.zeros(V, np.int32),
+                   tok_pos=np.zeros(self.n, np.int32), tok_cycle=np.zeros(self.n, np.int32))
+        self._check(self._lib.nomad_get_route_state(
+            self._h, _ptr(out["vptr"], C.c_int64), _ptr(out["unit"], C.c_int32), _ptr(out["nratings"], C.c_int32),
+            _ptr(out["count"], C.c_int32), _ptr(out["route_pos"], C.c_int32), _ptr(out["tok_pos"], C.c_int32),
+            _ptr(out["tok_cycle"], C.c_int32), None))
+        return out
+
+    def set_visit_counts(self, count) -> None:
+        c = np.ascontiguousarray(count, np.int32)
+        self._check(self._lib.nomad_set_visit_counts(self._h, _ptr(c, C.c_int32), c.size))
+
     def upload(self, train: Csc, test: Csc | None = None) -> None:
         for a, fn in ((train, self._lib.nomad_upload_train), (test, self._lib.nomad_upload_test)):
             if a is None:
@@ -451,6 +474,11 @@ class Context:
     def set_factors(self, W: np.ndarray | None, H: np.ndarray | None) -> None:
         Wc = None if W is None else np.ascontiguousarray(W, np.float64)
         Hc = None if H is None else np.ascontiguousarray(H, np.float64)
+        # the library reads m*k (n*k) doubles: a wrongly shaped array is refused here
+        if Wc is not None and Wc.shape != (self.m, self.k):
+            raise ValueError(f"W has shape {Wc.shape}, expected {(self.m, self.k)}")
+        if Hc is not None and Hc.shape != (self.n, self.k):
+            raise ValueError(f"H has shape {Hc.shape}, expected {(self.n, self.k)}")
         self._check(self._lib.nomad_set_factors(self._h, None if Wc is None else _ptr(Wc, C.c_double),
                                                 None if Hc is None else _ptr(Hc, C.c_double)))
 

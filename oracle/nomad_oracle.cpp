@@ -9,7 +9,9 @@
 //                    libmf's smalldata fixture converted to the format.
 //   orc_init_*       nb_main.cpp:70-98, 444-456, pool.cpp:32-38; pinned by an
 //                    independent MT19937-64 (checked against the standard's
-//                    10000th output) and the generate_canonical formula in
+//                    10000th output), the generate_canonical formula and a
+//                    Python restatement of libstdc++'s std::shuffle (two
+//                    swaps per draw, Lemire's bounded draw) in
 //                    tests/test_oracle.py.
 //   orc_step         closed form checked in tests.
 //   orc_sgd_strat    ARITH_REF: nb_main.cpp:514-544 restated; pinned by a
@@ -19,9 +21,17 @@
 //                    fused update and 64-lane tree; pinned against ARITH_REF
 //                    (agreement to rounding) and closed forms.
 //   orc_sgd_async    nb_main.cpp:415-578; pinned by invariants (update count =
-//                    nnz * epochs, T = 1 equals the serial replay bit-for-bit;
+//   orc_async_*      nnz * epochs, T = 1 equals the serial replay bit-for-bit;
 //                    bf16 at T = 1 equals a float32 serial replay rounded by
-//                    torch's bf16 conversion, bit-for-bit).
+//                    torch's bf16 conversion, bit-for-bit).  The segments
+//                    (nb_main.cpp:109-328): at T = 1, segments cut by update
+//                    budgets at arbitrary points equal one uninterrupted serial
+//                    replay bit-for-bit, with visit counts = visits (a count
+//                    reset by set_perm, or columns restarted out of order,
+//                    fails it); at T > 1 the per-thread counts sum to the
+//                    updates' column visits and no thread's count of a
+//                    column exceeds the column's completed cycles plus the
+//                    segments run.
 //   orc_sse          nb_main.cpp:601-667; pinned by numpy FP64.
 //   orc_loss_grad    sgd-mpi-omp.cpp:122-164 objective; pinned by the
 //                    reference DSGD binary's printed objective (oracle/_ref)
@@ -409,14 +419,15 @@ int64_t orc_sgd_strat(int storage, int arith, int rounding, uint32_t sr_seed, in
 }
 
 // ---------------------------------------------------------------------------
-// Asynchronous NOMAD with T CPU threads, one rank (nb_main.cpp:415-578).
+// Asynchronous NOMAD with T CPU threads, one rank (nb_main.cpp:415-578), as a
+// state kept across timeout segments (nb_main.cpp:109-328).
 namespace {
 
 struct Token {
     int col;
     int pos;
-    int64_t cycles;
-    std::vector<int> perm;
+    int64_t cycles;        // completed route cycles (pos wrapped to 0)
+    std::vector<int> perm;  // ColumnData::perm_ (pool.h:11-37)
 };
 
 struct TokQueue {
@@ -435,161 +446,271 @@ struct TokQueue {
     }
 };
 
+struct AsyncBase {
+    virtual ~AsyncBase() {}
+    virtual int segment(double seconds, int64_t max_updates, int64_t epochs, orc_async_stats* st) = 0;
+    virtual void get(double* W, double* H) const = 0;
+    virtual void counts(int32_t* out) const = 0;
+};
+
 template <class A>
-int async_run(int T, int m, int n, int k, const int64_t* colptr, const int32_t* rowidx, const double* val,
-              double* W, double* H, uint64_t seed, double lrate, double drate, double reg, double seconds,
-              int64_t epochs, orc_async_stats* st, int storage) {
-    const int per = m / T + ((m % T) > 0 ? 1 : 0);
-    // bf16 storage (A = float): every stored factor value is rounded to the
-    // nearest bf16 -- the initial factors, a row after each of its updates and
-    // a column at the end of each visit; arithmetic stays fp32
-    const bool bf16 = storage == ORC_BF16;
-    auto round_store = [&](A* v, int len) {
-        if (bf16)
-            for (int d = 0; d < len; ++d) v[d] = A(bf16_to_f(bf16_rne(float(v[d]))));
-    };
-    // per-thread CSC with local rows (nb_data.cpp:131-169)
+struct AsyncState : AsyncBase {
+    int T, m, n, k, storage;
+    double lrate, drate;
+    A regA;
+    // per-thread CSC with local rows (nb_data.cpp:131-169), W block and the
+    // thread's visit count per column (col_update_counts, nb_main.cpp:458-459)
     struct Part {
         int min_row = 0, rows = 0;
         std::vector<int64_t> off;
         std::vector<int> ri;
         std::vector<A> rv;
         std::vector<A> w;
-        std::vector<int> counts;
+        std::vector<int32_t> counts;
     };
-    std::vector<Part> parts(T);
-    for (int t = 0; t < T; ++t) {
-        Part& P = parts[t];
-        P.min_row = std::min(per * t, m);
-        const int max_row = std::min(per * (t + 1), m);
-        P.rows = std::max(0, max_row - P.min_row);
-        P.off.assign(size_t(n) + 1, 0);
-        P.w.resize(size_t(P.rows) * k);
-        for (int64_t e = 0; e < int64_t(P.rows) * k; ++e) P.w[e] = A(W[int64_t(P.min_row) * k + e]);
-        for (int r = 0; r < P.rows; ++r) round_store(&P.w[int64_t(r) * k], k);
-        P.counts.assign(n, 0);
+    std::vector<Part> parts;
+    std::vector<Token> toks;
+    std::vector<std::vector<A>> hv;
+    std::vector<TokQueue> queues;
+    std::vector<Token*> parked;
+    std::mt19937_64 rng;  // NomadBody::rng, mt19937_64(seed + rank * 131 + 139) (nb_init.cpp:219)
+
+    // bf16 storage (A = float): every stored factor value is rounded to the
+    // nearest bf16 -- the initial factors, a row after each of its updates and
+    // a column at the end of each visit; arithmetic stays fp32
+    void round_store(A* v, int len) const {
+        if (storage == ORC_BF16)
+            for (int d = 0; d < len; ++d) v[d] = A(bf16_to_f(bf16_rne(float(v[d]))));
     }
-    // one pass: count, then place (rows stay ascending inside a column)
-    for (int j = 0; j < n; ++j)
-        for (int64_t o = colptr[j]; o < colptr[j + 1]; ++o) parts[rowidx[o] / per].off[j + 1]++;
-    for (int t = 0; t < T; ++t) {
-        Part& P = parts[t];
-        for (int j = 0; j < n; ++j) P.off[j + 1] += P.off[j];
-        P.ri.resize(P.off[n]);
-        P.rv.resize(P.off[n]);
-    }
-    {
-        std::vector<std::vector<int64_t>> fill(T);
-        for (int t = 0; t < T; ++t) fill[t].assign(parts[t].off.begin(), parts[t].off.end() - 1);
+
+    AsyncState(int storage_, int T_, int m_, int n_, int k_, const int64_t* colptr, const int32_t* rowidx,
+               const double* val, const double* W, const double* H, uint64_t seed, double lrate_, double drate_,
+               double reg)
+        : T(T_), m(m_), n(n_), k(k_), storage(storage_), lrate(lrate_), drate(drate_),
+          regA(storage_ == ORC_FP64 ? A(reg) : A(float(reg))), parts(T_), toks(n_), hv(n_), queues(T_),
+          rng(seed + 139) {
+        const int per = m / T + ((m % T) > 0 ? 1 : 0);  // nb_data.cpp:83
+        for (int t = 0; t < T; ++t) {
+            Part& P = parts[t];
+            P.min_row = std::min(per * t, m);
+            const int max_row = std::min(per * (t + 1), m);
+            P.rows = std::max(0, max_row - P.min_row);
+            P.off.assign(size_t(n) + 1, 0);
+            P.w.resize(size_t(P.rows) * k);
+            for (int64_t e = 0; e < int64_t(P.rows) * k; ++e) P.w[e] = A(W[int64_t(P.min_row) * k + e]);
+            for (int r = 0; r < P.rows; ++r) round_store(&P.w[int64_t(r) * k], k);
+            P.counts.assign(n, 0);
+        }
+        // one pass: count, then place (rows stay ascending inside a column)
         for (int j = 0; j < n; ++j)
-            for (int64_t o = colptr[j]; o < colptr[j + 1]; ++o) {
-                const int t = rowidx[o] / per;
-                const int64_t d = fill[t][j]++;
-                parts[t].ri[d] = rowidx[o] - parts[t].min_row;
-                parts[t].rv[d] = A(val[o]);
-            }
-    }
-    std::vector<Token> toks(n);
-    std::vector<std::vector<A>> hv(n, std::vector<A>(k));
-    std::vector<TokQueue> queues(T);
-    std::mt19937_64 rng_temp(seed + 137);
-    for (int j = 0; j < n; ++j) {
-        Token& tk = toks[j];
-        tk.col = j;
-        tk.pos = 0;
-        tk.cycles = 0;
-        tk.perm.resize(T);
-        for (int i = 0; i < T; ++i) tk.perm[i] = i;
-        std::shuffle(tk.perm.begin(), tk.perm.end(), rng_temp);
-        for (int d = 0; d < k; ++d) hv[j][d] = A(H[int64_t(j) * k + d]);
-        round_store(hv[j].data(), k);
-        queues[tk.perm[tk.pos]].push(&tk);
-    }
-    const A regA = storage == ORC_FP64 ? A(reg) : A(float(reg));
-    std::atomic<bool> stop{false};
-    std::atomic<int> parked{0};
-    std::atomic<int64_t> total_updates{0}, total_fail{0};
-    const auto t0 = std::chrono::steady_clock::now();
-    auto worker = [&](int t) {
-        Part& P = parts[t];
-        int64_t nupd = 0, nfail = 0;
-        while (!stop.load(std::memory_order_relaxed)) {
-            Token* tk;
-            if (queues[t].try_pop(tk)) {
-                const int j = tk->col;
-                const A step = A(orc_step(storage, lrate, drate, P.counts[j]));
-                A* col = hv[j].data();
-                for (int64_t o = P.off[j]; o < P.off[j + 1]; ++o) {
-                    A* row = &P.w[int64_t(P.ri[o]) * k];
-                    ref_update(row, col, k, P.rv[o], step, regA);
-                    round_store(row, k);
-                    ++nupd;
+            for (int64_t o = colptr[j]; o < colptr[j + 1]; ++o) parts[rowidx[o] / per].off[j + 1]++;
+        for (int t = 0; t < T; ++t) {
+            Part& P = parts[t];
+            for (int j = 0; j < n; ++j) P.off[j + 1] += P.off[j];
+            P.ri.resize(P.off[n]);
+            P.rv.resize(P.off[n]);
+        }
+        {
+            std::vector<std::vector<int64_t>> fill(T);
+            for (int t = 0; t < T; ++t) fill[t].assign(parts[t].off.begin(), parts[t].off.end() - 1);
+            for (int j = 0; j < n; ++j)
+                for (int64_t o = colptr[j]; o < colptr[j + 1]; ++o) {
+                    const int t = rowidx[o] / per;
+                    const int64_t d = fill[t][j]++;
+                    parts[t].ri[d] = rowidx[o] - parts[t].min_row;
+                    parts[t].rv[d] = A(val[o]);
                 }
-                round_store(col, k);
-                P.counts[j]++;
-                tk->pos++;
-                if (tk->pos >= T) {
-                    tk->pos = 0;
-                    tk->cycles++;
-                    if (epochs > 0 && tk->cycles >= epochs) {
-                        if (parked.fetch_add(1) + 1 == n) stop = true;
-                        continue;
+        }
+        // initial placement: set_perm with mt19937_64(seed + rank + 137), columns
+        // in order, each pushed to the queue of its first thread (nb_main.cpp:70-97)
+        std::mt19937_64 rng_temp(seed + 137);
+        for (int j = 0; j < n; ++j) {
+            Token& tk = toks[j];
+            tk.col = j;
+            tk.pos = 0;
+            tk.cycles = 0;
+            tk.perm.resize(T);
+            for (int i = 0; i < T; ++i) tk.perm[i] = i;
+            std::shuffle(tk.perm.begin(), tk.perm.end(), rng_temp);
+            hv[j].resize(k);
+            for (int d = 0; d < k; ++d) hv[j][d] = A(H[int64_t(j) * k + d]);
+            round_store(hv[j].data(), k);
+            queues[tk.perm[tk.pos]].push(&tk);
+        }
+    }
+
+    int segment(double seconds, int64_t max_updates, int64_t epochs, orc_async_stats* st) override {
+        int64_t target = INT64_MAX;
+        if (epochs > 0) {
+            int64_t mn = INT64_MAX;
+            for (const Token& t : toks) mn = std::min(mn, t.cycles);
+            target = (n ? mn : 0) + epochs;
+        }
+        std::atomic<bool> stop{false};
+        std::atomic<int> nparked{0};
+        std::atomic<int64_t> total_updates{0}, total_fail{0};
+        std::mutex park_mu;
+        const auto t0 = std::chrono::steady_clock::now();
+        auto worker = [&](int t) {
+            Part& P = parts[t];
+            int64_t nupd = 0, nfail = 0, npoll = 0;
+            // the stop flag is read before each pop, so a visit always completes
+            // (flag_train_stop, nb_main.cpp:472, 480-570)
+            while (!stop.load(std::memory_order_relaxed)) {
+                Token* tk;
+                if (queues[t].try_pop(tk)) {
+                    const int j = tk->col;
+                    const A step = A(orc_step(storage, lrate, drate, P.counts[j]));  // nb_main.cpp:513-515
+                    A* col = hv[j].data();
+                    int64_t vis = 0;
+                    for (int64_t o = P.off[j]; o < P.off[j + 1]; ++o) {
+                        A* row = &P.w[int64_t(P.ri[o]) * k];
+                        ref_update(row, col, k, P.rv[o], step, regA);
+                        round_store(row, k);
+                        ++vis;
                     }
+                    round_store(col, k);
+                    nupd += vis;
+                    P.counts[j]++;  // nb_main.cpp:544
+                    tk->pos++;      // nb_main.cpp:552-565 at one rank
+                    bool park = false;
+                    if (tk->pos >= T) {
+                        tk->pos = 0;
+                        tk->cycles++;
+                        park = tk->cycles >= target;
+                    }
+                    if (park) {
+                        {
+                            std::lock_guard<std::mutex> g(park_mu);
+                            parked.push_back(tk);
+                        }
+                        if (nparked.fetch_add(1) + 1 == n) stop = true;
+                    } else {
+                        queues[tk->perm[tk->pos % T]].push(tk);
+                    }
+                    if (max_updates > 0 && total_updates.fetch_add(vis) + vis >= max_updates) stop = true;
+                } else {
+                    ++nfail;
+                    std::this_thread::yield();
                 }
-                queues[tk->perm[tk->pos % T]].push(tk);
-            } else {
-                ++nfail;
-                std::this_thread::yield();
+                if (seconds > 0 && (++npoll & 63) == 0) {
+                    const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                    if (el >= seconds) stop = true;
+                }
             }
-            if (seconds > 0 && (nupd & 1023) == 0) {
-                const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-                if (el >= seconds) stop = true;
+            if (max_updates <= 0) total_updates += nupd;
+            total_fail += nfail;
+        };
+        if (T == 1) {
+            worker(0);
+        } else {
+            std::vector<std::thread> th;
+            for (int t = 0; t < T; ++t) th.emplace_back(worker, t);
+            if (seconds > 0) {
+                while (!stop) {
+                    std::this_thread::sleep_for(std::chrono::milliseconds(1));
+                    const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                    if (el >= seconds) stop = true;
+                }
             }
+            for (auto& x : th) x.join();
         }
-        total_updates += nupd;
-        total_fail += nfail;
-    };
-    std::vector<std::thread> th;
-    for (int t = 0; t < T; ++t) th.emplace_back(worker, t);
-    if (seconds > 0) {
-        while (!stop) {
-            std::this_thread::sleep_for(std::chrono::milliseconds(1));
-            const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-            if (el >= seconds) stop = true;
+        const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        // gather every column, queues in thread order (nb_main.cpp:140-178), then
+        // parked ones; each starts a fresh route: set_perm (pool.cpp:32-38) with
+        // the member generator, pushed to its first thread (nb_main.cpp:313-321)
+        std::vector<Token*> all;
+        for (int t = 0; t < T; ++t) {
+            Token* tk;
+            while (queues[t].try_pop(tk)) all.push_back(tk);
         }
+        for (Token* tk : parked) all.push_back(tk);
+        parked.clear();
+        for (Token* tk : all) {
+            tk->pos = 0;
+            for (int i = 0; i < T; ++i) tk->perm[i] = i;
+            std::shuffle(tk->perm.begin(), tk->perm.end(), rng);
+            queues[tk->perm[0]].push(tk);
+        }
+        int64_t mn = INT64_MAX;
+        for (const Token& t : toks) mn = std::min(mn, t.cycles);
+        if (st) {
+            st->updates = total_updates;
+            st->failures = total_fail;
+            st->seconds = el;
+            st->min_cycles = n ? mn : 0;
+        }
+        return int(all.size()) == n ? 0 : -2;
     }
-    for (auto& x : th) x.join();
-    const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    for (int t = 0; t < T; ++t)
-        for (int64_t e = 0; e < int64_t(parts[t].rows) * k; ++e)
-            W[int64_t(parts[t].min_row) * k + e] = double(parts[t].w[e]);
-    int64_t mn = INT64_MAX;
-    for (int j = 0; j < n; ++j) {
-        for (int d = 0; d < k; ++d) H[int64_t(j) * k + d] = double(hv[j][d]);
-        mn = std::min(mn, toks[j].cycles);
+
+    void get(double* W, double* H) const override {
+        if (W)
+            for (int t = 0; t < T; ++t)
+                for (int64_t e = 0; e < int64_t(parts[t].rows) * k; ++e)
+                    W[int64_t(parts[t].min_row) * k + e] = double(parts[t].w[e]);
+        if (H)
+            for (int j = 0; j < n; ++j)
+                for (int d = 0; d < k; ++d) H[int64_t(j) * k + d] = double(hv[j][d]);
     }
-    if (st) {
-        st->updates = total_updates;
-        st->failures = total_fail;
-        st->seconds = el;
-        st->min_cycles = n ? mn : 0;
+
+    void counts(int32_t* out) const override {
+        for (int t = 0; t < T; ++t) std::copy(parts[t].counts.begin(), parts[t].counts.end(), out + int64_t(t) * n);
     }
+};
+
+}  // namespace
+
+struct orc_async {
+    AsyncBase* impl;
+};
+
+orc_async* orc_async_create(int storage, int T, int m, int n, int k, const int64_t* colptr, const int32_t* rowidx,
+                            const double* val, const double* W, const double* H, uint64_t seed, double lrate,
+                            double drate, double reg) {
+    if (T <= 0 || m <= 0 || n <= 0 || k <= 0) return nullptr;
+    AsyncBase* b = nullptr;
+    if (storage == ORC_FP32 || storage == ORC_BF16)
+        b = new AsyncState<float>(storage, T, m, n, k, colptr, rowidx, val, W, H, seed, lrate, drate, reg);
+    else if (storage == ORC_FP64)
+        b = new AsyncState<double>(storage, T, m, n, k, colptr, rowidx, val, W, H, seed, lrate, drate, reg);
+    else
+        return nullptr;
+    return new orc_async{b};
+}
+
+int orc_async_segment(orc_async* s, double seconds, int64_t max_updates, int64_t epochs, orc_async_stats* stats) {
+    if (!s || (!(seconds > 0) && max_updates <= 0 && epochs <= 0)) return -1;
+    return s->impl->segment(seconds, max_updates, epochs, stats);
+}
+
+int orc_async_get(const orc_async* s, double* W, double* H) {
+    if (!s) return -1;
+    s->impl->get(W, H);
     return 0;
 }
 
-}  // namespace
+int orc_async_counts(const orc_async* s, int32_t* counts) {
+    if (!s || !counts) return -1;
+    s->impl->counts(counts);
+    return 0;
+}
+
+void orc_async_destroy(orc_async* s) {
+    if (!s) return;
+    delete s->impl;
+    delete s;
+}
 
 int orc_sgd_async(int storage, int T, int m, int n, int k, const int64_t* colptr, const int32_t* rowidx,
                   const double* val, double* W, double* H, uint64_t seed, double lrate, double drate,
                   double reg, double seconds, int64_t epochs, orc_async_stats* stats) {
-    if (T <= 0 || m <= 0 || n <= 0 || k <= 0 || (!(seconds > 0) && epochs <= 0)) return -1;
-    if (storage == ORC_FP32 || storage == ORC_BF16)
-        return async_run<float>(T, m, n, k, colptr, rowidx, val, W, H, seed, lrate, drate, reg, seconds, epochs,
-                                stats, storage);
-    if (storage == ORC_FP64)
-        return async_run<double>(T, m, n, k, colptr, rowidx, val, W, H, seed, lrate, drate, reg, seconds, epochs,
-                                 stats, storage);
-    return -1;
+    if (!(seconds > 0) && epochs <= 0) return -1;
+    orc_async* s = orc_async_create(storage, T, m, n, k, colptr, rowidx, val, W, H, seed, lrate, drate, reg);
+    if (!s) return -1;
+    const int rc = orc_async_segment(s, seconds, 0, epochs, stats);
+    orc_async_get(s, W, H);
+    orc_async_destroy(s);
+    return rc;
 }
 
 // ---------------------------------------------------------------------------

@@ -82,6 +82,26 @@ int orc_sgd_async(int storage, int T, int m, int n, int k, const int64_t* colptr
                   const double* val, double* W, double* H, uint64_t seed, double lrate, double drate,
                   double reg, double seconds, int64_t epochs, orc_async_stats* stats);
 
+/* The same run as a state kept across timeout segments (nb_main.cpp:109-328):
+ * each thread's W block and its visit count per column (col_update_counts,
+ * nb_main.cpp:458-459, 544) persist; a segment stops after `seconds` (> 0),
+ * or once `max_updates` (> 0) updates have been made (each thread completes
+ * the visit it is in: at T = 1 the segment ends with the visit that reaches
+ * the budget), or when every column has completed `epochs` (> 0) more route
+ * cycles than the least advanced one had at the start; then every column is
+ * gathered (queues in thread order, then parked columns) and given a fresh
+ * route with set_perm from mt19937_64(seed + 139) (pool.cpp:32-38,
+ * nb_init.cpp:219, nb_main.cpp:313-321).  orc_sgd_async is one segment of a
+ * fresh state.  counts[t * n + j]: thread t's visits of column j. */
+typedef struct orc_async orc_async;
+orc_async* orc_async_create(int storage, int T, int m, int n, int k, const int64_t* colptr, const int32_t* rowidx,
+                            const double* val, const double* W, const double* H, uint64_t seed, double lrate,
+                            double drate, double reg);
+int orc_async_segment(orc_async* s, double seconds, int64_t max_updates, int64_t epochs, orc_async_stats* stats);
+int orc_async_get(const orc_async* s, double* W, double* H);
+int orc_async_counts(const orc_async* s, int32_t* counts);
+void orc_async_destroy(orc_async* s);
+
 /* --- evaluation (nb_main.cpp:601-667) -------------------------------------- *
  * sum over ratings of (<h_j, w_i> - r)^2 in double, the dot in `storage`
  * precision with `arith` order. */

@@ -37,8 +37,12 @@ _SYMS = {
     "orc_sgd_async": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _i64p, _i32p, _f64p, _f64p, _f64p,
                                 C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64,
                                 C.POINTER(AsyncStats)]),
-    "orc_dat_part_info": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _i32p, _i32p, _i64p, _i32p, _i32p, _i64p]),
-    "orc_dat_part_read": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _i32p, _i32p, _f64p]),
+    "orc_async_create": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _i64p, _i32p, _f64p, _f64p,
+                                      _f64p, C.c_uint64, C.c_double, C.c_double, C.c_double]),
+    "orc_async_segment": (C.c_int, [C.c_void_p, C.c_double, C.c_int64, C.c_int64, C.POINTER(AsyncStats)]),
+    "orc_async_get": (C.c_int, [C.c_void_p, _f64p, _f64p]),
+    "orc_async_counts": (C.c_int, [C.c_void_p, _i32p]),
+    "orc_async_destroy": (None, [C.c_void_p]),
     "orc_bf16_store_pair": (None, [C.c_float, C.c_float, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                    C.POINTER(C.c_float)]),
     "orc_sse": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _i64p, _i32p, _f64p, _f64p, _f64p]),
@@ -142,6 +146,48 @@ def sgd_async(a, W, H, *, storage=FP32, T=4, seed=12345, lrate=0.001, drate=0.1,
     if rc != 0:
         raise RuntimeError("orc_sgd_async failed")
     return W, H, st
+
+
+class AsyncRun:
+    """The async restatement kept across timeout segments (orc_async_*):
+    segment() runs one segment (seconds, an update budget or epochs) and
+    re-permutes every column's route afterwards."""
+
+    def __init__(self, a, W, H, *, storage=FP32, T=4, seed=12345, lrate=0.001, drate=0.1, reg=0.05):
+        colptr, rowidx, val = _csc(a)
+        W = np.ascontiguousarray(W, np.float64)
+        H = np.ascontiguousarray(H, np.float64)
+        self.m, self.n, self.k, self.T = a.m, a.n, W.shape[1], T
+        self._h = lib().orc_async_create(storage, T, a.m, a.n, self.k, _p(colptr, C.c_int64), _p(rowidx, C.c_int32),
+                                         _p(val, C.c_double), _p(W, C.c_double), _p(H, C.c_double), seed, lrate,
+                                         drate, reg)
+        if not self._h:
+            raise RuntimeError("orc_async_create failed")
+
+    def segment(self, seconds=0.0, max_updates=0, epochs=0):
+        st = AsyncStats()
+        if lib().orc_async_segment(self._h, seconds, max_updates, epochs, C.byref(st)) != 0:
+            raise RuntimeError("orc_async_segment failed")
+        return st
+
+    def factors(self):
+        W = np.zeros((self.m, self.k))
+        H = np.zeros((self.n, self.k))
+        lib().orc_async_get(self._h, _p(W, C.c_double), _p(H, C.c_double))
+        return W, H
+
+    def counts(self):
+        c = np.zeros((self.T, self.n), np.int32)
+        lib().orc_async_counts(self._h, _p(c, C.c_int32))
+        return c
+
+    def close(self):
+        if self._h:
+            lib().orc_async_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 def sse(a, W, H, *, storage=FP64, arith=ARITH_REF):

@@ -162,17 +162,94 @@ def test_async_eight_waves_per_workgroup(gpu, small, storage):
     assert abs(gpu_rmse - cpu_rmse) <= 0.005 * cpu_rmse, (gpu_rmse, cpu_rmse)
 
 
-def test_async_timeout_segments(gpu, small):
+def _check_segment_counts(rs, prev_count, prev_cycle, updates):
+    """After a timed segment: each (column, workgroup) pair's visit count went
+    up by the column's completed route cycles in the segment, plus one if the
+    pair's hop came before the column's next hop in the segment's route; and
+    the pairs' new visits times their ratings are exactly the segment's
+    updates (every visit is whole and counted)."""
+    G = np.diff(rs["vptr"])
+    dcyc = np.repeat(rs["tok_cycle"] - prev_cycle, G)
+    part = (rs["route_pos"] < np.repeat(rs["tok_pos"], G)).astype(np.int32)
+    exp = prev_count + dcyc + part
+    bad = np.nonzero(rs["count"] != exp)[0]
+    assert bad.size == 0, (bad[:5], rs["count"][bad[:5]], exp[bad[:5]])
+    assert int(((rs["count"] - prev_count).astype(np.int64) * rs["nratings"]).sum()) == updates
+    return part.any()
+
+
+@pytest.mark.parametrize("storage,rounding", [("fp32", "nearest"), ("bf16", "nearest"), ("fp64", "nearest")])
+def test_async_timed_segments_count_every_visit(gpu, small, storage, rounding):
+    """Timeout segments (nb_main.cpp:109-328) count steps as the reference
+    does, one count per worker and column (nb_main.cpp:458-459, 513-515,
+    544), kept per (column, workgroup) pair here: after every segment the
+    device counts equal the visits the routes and token records show,
+    partial route cycles included, and they account for every update; after
+    eight segments the test RMSE is within 0.5 % of the oracle's async
+    restatement (4 threads, per-thread counts, set_perm after each segment)
+    run with the same per-segment update budgets."""
+    tr, te = small
+    k = 16
+    hyp = dict(lrate=0.01, drate=0.1, reg=0.05)  # steps that move with the count
+    W0, H0 = gpu.init_factors(tr.m, tr.n, k, 4, 12345)
+    budgets = []
+    partial = 0
+    with gpu.Context(tr.m, tr.n, k, storage=storage, rounding=rounding, mode="async", nworkers=1536, seed=12345,
+                     **hyp) as ctx:
+        ctx.upload(tr, te)
+        ctx.set_factors(W0, H0)
+        rs = ctx.route_state()
+        assert (rs["count"] == 0).all() and (rs["tok_pos"] == 0).all()
+        prev_count, prev_cycle = rs["count"].copy(), rs["tok_cycle"].copy()
+        for s in range(8):
+            st = ctx.train(seconds=0.0004)
+            assert st.updates > 0
+            budgets.append(int(st.updates))
+            rs = ctx.route_state()
+            partial += _check_segment_counts(rs, prev_count, prev_cycle, st.updates)
+            prev_count, prev_cycle = rs["count"].copy(), rs["tok_cycle"].copy()
+        ev = ctx.eval()
+    assert partial >= 4, "segments should end inside route cycles"
+    gpu_rmse = np.sqrt(ev.test_sse / ev.test_count)
+    ost = {"fp32": O.FP32, "bf16": O.BF16, "fp64": O.FP64}[storage]
+    run = O.AsyncRun(tr, W0, H0, storage=ost, T=4, seed=12345, **hyp)
+    for b in budgets:
+        run.segment(max_updates=b)
+    Wc, Hc = run.factors()
+    run.close()
+    cpu_rmse = np.sqrt(O.sse(te, Wc, Hc, storage=O.FP64) / te.nnz)
+    print(f"[timed segments {storage}] budgets {budgets} gpu {gpu_rmse:.6f} cpu {cpu_rmse:.6f}", flush=True)
+    assert abs(gpu_rmse - cpu_rmse) <= 0.005 * cpu_rmse, (gpu_rmse, cpu_rmse)
+
+
+def test_async_epoch_counts_equal_cycles(gpu, small):
+    """In epoch mode every pair's visit count equals the epochs run."""
     tr, te = small
     k = 16
     W0, H0 = gpu.init_factors(tr.m, tr.n, k, 4, 1)
-    with _ctx(gpu, tr, te, k, storage="fp32", mode="async", nworkers=32) as ctx:
+    with _ctx(gpu, tr, te, k, storage="fp32", mode="async", nworkers=256) as ctx:
         ctx.set_factors(W0, H0)
-        st1 = ctx.train(seconds=0.05)
-        st2 = ctx.train(seconds=0.05)
-        ev = ctx.eval()
-    assert st1.updates > 0 and st2.updates > 0
-    assert np.isfinite(ev.test_sse)
+        for ep in range(3):
+            ctx.train(epochs=1)
+            rs = ctx.route_state()
+            assert (rs["count"] == ep + 1).all() and (rs["tok_cycle"] == ep + 1).all()
+            assert (rs["tok_pos"] == 0).all()
+
+
+def test_async_ptoken_caps_resident_columns(gpu, small):
+    """--ptoken caps the columns resident per workgroup: with a cap of 2 the
+    claims never hold more than 2 slots, and training still completes every
+    epoch exactly."""
+    tr, te = small
+    k = 16
+    W0, H0 = gpu.init_factors(tr.m, tr.n, k, 4, 1)
+    with _ctx(gpu, tr, te, k, storage="fp32", mode="async", nworkers=512, ptoken=2) as ctx:
+        ctx.set_factors(W0, H0)
+        st = ctx.train(epochs=2)
+        c = ctx.counters()
+    assert st.updates == 2 * tr.nnz
+    # resident_sum / claims = mean resident columns seen at a claim attempt
+    assert c["resident_sum"] <= 2 * c["claims"]
 
 
 def test_eval_matches_oracle(gpu, small):

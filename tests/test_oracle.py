@@ -80,6 +80,60 @@ def test_init_perms_are_permutations():
     assert not np.array_equal(P, O.init_perms(50, 6, 12346))
 
 
+M64 = (1 << 64) - 1
+
+
+def _uid(g, lo, hi):
+    """libstdc++'s uniform_int_distribution{lo, hi} on a 64-bit engine:
+    Lemire's multiply-and-shift draw over 128 bits, redrawn while the low
+    half falls below (2^64 - range) mod range."""
+    rng = hi - lo + 1
+    prod = g() * rng
+    if (prod & M64) < rng:
+        thr = ((1 << 64) - rng) % rng
+        while (prod & M64) < thr:
+            prod = g() * rng
+    return (prod >> 64) + lo
+
+
+def _shuffle(v, g):
+    """libstdc++'s std::shuffle for a 64-bit engine and a short range: an
+    even-length range first swaps element 1 with a draw in {0, 1}; then two
+    elements per draw, positions x / (r + 1) and x % (r + 1) of one draw
+    x in [0, r (r + 1)) with r = i + 1."""
+    n = len(v)
+    if n == 0:
+        return
+    assert M64 // n >= n
+    i = 1
+    if n % 2 == 0:
+        r = _uid(g, 0, 1)
+        v[i], v[r] = v[r], v[i]
+        i += 1
+    while i != n:
+        r = i + 1
+        x = _uid(g, 0, r * (r + 1) - 1)
+        p1, p2 = x // (r + 1), x % (r + 1)
+        v[i], v[p1] = v[p1], v[i]
+        i += 1
+        v[i], v[p2] = v[p2], v[i]
+        i += 1
+
+
+@pytest.mark.parametrize("T,seed", [(1, 5), (2, 12345), (3, 7), (6, 12345), (7, 99), (64, 1), (256, 12345)])
+def test_init_perms_match_libstdcxx_shuffle(T, seed):
+    """orc_init_perms (set_perm with mt19937_64(seed + 137), pool.cpp:32-38,
+    nb_main.cpp:79-87) equals the restated shuffle draw for draw; a wrong
+    seed offset, a missing first swap or a one-swap-per-draw shuffle fails."""
+    n = 40
+    P = O.init_perms(n, T, seed)
+    g = MT64(seed + 137)
+    for j in range(n):
+        v = list(range(T))
+        _shuffle(v, g)
+        assert P[j].tolist() == v, (T, seed, j)
+
+
 def test_step_closed_form():
     for c in range(5):
         exp = 0.001 * 1.5 / (1.0 + 0.1 * (c + 1) ** 1.5)
@@ -222,6 +276,63 @@ def test_async_bf16_one_thread_matches_float32_replay():
     assert not np.any(Wa.astype(np.float32).view(np.uint32) & 0xFFFF)
     Wf, _, _ = O.sgd_async(tr, W0, H0, storage=O.FP32, T=1, seed=4, epochs=2, **hyp)
     assert not np.array_equal(Wf, Wa)  # the rounding is really applied
+
+
+@pytest.mark.parametrize("storage", [O.FP32, O.FP64])
+def test_async_segments_one_thread_is_serial(storage):
+    """Timeout segments at T = 1 (nb_main.cpp:109-328): segments cut by update
+    budgets mid-epoch and mid-column leave the column rotation and the visit
+    counts running, so after three epochs' worth of updates the result equals
+    three uninterrupted epochs of the serial order bit for bit, and thread 0
+    counts three visits of every column.  Restarting each segment at column
+    0, or a count reset by set_perm, would change the steps and fail it."""
+    tr, _ = lib.synth(lib.preset(1, 0.1))
+    W0, H0 = lib.init_factors(tr.m, tr.n, 8, 1, 4)
+    hyp = dict(lrate=0.01, drate=0.1, reg=0.05)
+    run = O.AsyncRun(tr, W0, H0, storage=storage, T=1, seed=4, **hyp)
+    total = 3 * tr.nnz
+    budgets = [tr.nnz // 3 + 7, tr.nnz, tr.nnz // 2 + 1]
+    done = 0
+    for b in budgets:
+        st = run.segment(max_updates=b)
+        assert b <= st.updates <= b + int(np.diff(tr.colptr).max())
+        done += st.updates
+    st = run.segment(max_updates=total - done)
+    done += st.updates
+    assert done == total
+    Wa, Ha = run.factors()
+    Ws, Hs = O.sgd_strat(tr, W0, H0, storage=storage, arith=O.ARITH_REF, B=1, epochs=3, **hyp)
+    assert np.array_equal(Wa, Ws) and np.array_equal(Ha, Hs)
+    c = run.counts()
+    nonempty = np.diff(tr.colptr) > 0
+    last = np.nonzero(nonempty)[0].max()
+    # columns after the last non-empty one are visited (empty) once the next
+    # segment starts; every column up to it has exactly three visits
+    assert (c[0, : last + 1] == 3).all()
+    run.close()
+
+
+def test_async_segments_threads_counts_match_updates():
+    """At T > 1 each thread's visit count of a column, times the column's
+    ratings in that thread's rows, summed over threads and columns, equals
+    the updates of all segments; and a column's count at any thread lies
+    between its completed route cycles and that plus the segments run."""
+    tr, _ = lib.synth(lib.preset(1, 0.1))
+    T = 4
+    W0, H0 = lib.init_factors(tr.m, tr.n, 8, T, 4)
+    run = O.AsyncRun(tr, W0, H0, storage=O.FP32, T=T, seed=4, **HYP)
+    ups = 0
+    for b in (tr.nnz // 2, tr.nnz, tr.nnz // 3):
+        ups += run.segment(max_updates=b).updates
+    ups += run.segment(epochs=1).updates
+    c = run.counts()
+    per = -(-tr.m // T)
+    part = np.zeros((T, tr.n), np.int64)
+    r, col, _ = tr.to_triplets()
+    np.add.at(part, (r // per, col), 1)
+    assert int((c.astype(np.int64) * part).sum()) == ups
+    assert c.min() >= 1 and c.max() - c.min() <= 4 + 2
+    run.close()
 
 
 def test_async_threads_update_count():

@@ -134,3 +134,28 @@ def test_driver_rejects_bad_options(tmp_path):
     assert r.returncode != 0 and "train.dat" in r.stderr
     r = run_driver("--path", str(tmp_path), "--ptoken", "-1")  # rejected before any file is read
     assert r.returncode != 0 and "cannot open" not in r.stderr
+
+
+@pytest.mark.parametrize("nparts", [1, 4, 13])
+def test_global_csc_helper_merges_reference_parts(tmp_path, nparts):
+    """include/nomad_csc.h merges the reference's per-thread parts (local row
+    ids, Code/nomad/nomad_body.h:34-46) into the global CSC the library takes:
+    equal to the whole file read at once, for 1, 4 and 13 parts."""
+    a = smalldata()
+    p = tmp_path / "small.dat"
+    lib.write_dat(p, a)
+    exe = tmp_path / "csc_parts"
+    subprocess.run(["g++", "-std=c++17", "-O1", f"-I{ROOT / 'include'}", f"-I{ROOT / 'oracle'}",
+                    str(ROOT / "tests" / "csc_parts_main.cpp"), str(ROOT / "oracle" / "liborc.so"),
+                    f"-Wl,-rpath,{ROOT / 'oracle'}", "-o", str(exe)], check=True)
+    out = tmp_path / "g.bin"
+    subprocess.run([str(exe), str(p), str(nparts), str(out)], check=True)
+    raw = out.read_bytes()
+    n, nnz = a.n, a.nnz
+    colptr = np.frombuffer(raw, np.int64, n + 1)
+    rowidx = np.frombuffer(raw, np.int32, nnz, 8 * (n + 1))
+    val = np.frombuffer(raw, np.float64, nnz, 8 * (n + 1) + 4 * nnz)
+    b = lib.read_dat(p)
+    np.testing.assert_array_equal(colptr, b.colptr)
+    np.testing.assert_array_equal(rowidx, b.rowidx)
+    np.testing.assert_array_equal(val, b.val)
