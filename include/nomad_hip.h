@@ -200,6 +200,17 @@ int nomad_autotune_batch(const nomad_batch_problem* probs, int32_t count, int32_
  * FP64 (the dot product runs in the storage's arithmetic type). */
 int nomad_eval(nomad_ctx* ctx, nomad_eval_stats* stats);
 
+/* nomad_eval without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same result as nomad_eval, bit for bit; the per-wave partial
+ * sums go to a fixed 8 MiB device array of the library (one copy per GPU,
+ * held until the process ends) instead of a buffer the context allocates,
+ * so the training and the test set may each hold at most 536,868,864
+ * ratings ((2^20 - 4) x 512; larger sets fail with NOMAD_EINVAL).  Calls
+ * that overlap on one GPU (several host threads) race on that array. */
+int nomad_eval_noalloc(nomad_ctx* ctx, nomad_eval_stats* stats);
+
 /* Row blocks of NOMAD_FLAG_BALANCE: bounds[0..P] (bounds[0] = 0, bounds[P] =
  * m) split rows 0..m-1 into P contiguous non-empty blocks whose largest
  * training-rating count relative to the block's capacity cap[b] (in (0, 1];

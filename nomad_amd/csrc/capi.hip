@@ -142,6 +142,39 @@ void coh_free(T*& p) {
     p = nullptr;
 }
 
+// Device memory of nomad_eval_noalloc: the per-wave partial sums of the SSE
+// kernel, a fixed array in the library's code object instead of memory the
+// context allocates.  The runtime places one copy on each GPU the library
+// runs on, kept until the process ends.  In host code `scratch` only names
+// the symbol: its device address is looked up once per device, with that
+// device current (nomad_create), and only that address is handed to kernels
+// and copies.  Every call on a GPU shares the array, so calls that overlap on
+// one GPU race; each call writes every partial it reads before reading it.
+constexpr int64_t kScratchPartials = int64_t(1) << 20;  // 8 MiB of doubles
+}  // namespace
+static __device__ __align__(256) double scratch[kScratchPartials];
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr;  // per device ordinal: scratch's address there
+
+// Looks up scratch's address on `device` (which must be current) once.
+int resolve_scratch(int device) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (int(g_scratch_ptr.size()) <= device) g_scratch_ptr.resize(size_t(device) + 1, nullptr);
+    if (g_scratch_ptr[size_t(device)]) return NOMAD_OK;
+    double* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(reinterpret_cast<void**>(&p), HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(NOMAD_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[size_t(device)] = p;
+    return NOMAD_OK;
+}
+
+double* scratch_on(int device) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    return device >= 0 && device < int(g_scratch_ptr.size()) ? g_scratch_ptr[size_t(device)] : nullptr;
+}
+
 struct DevCsc {
     int64_t nnz = -1;  // -1 = not uploaded
     int64_t* colptr = nullptr;
@@ -1114,16 +1147,31 @@ int train_batch(nomad_ctx* const* cs, int32_t count, int64_t epochs, int32_t wav
     return NOMAD_OK;
 }
 
-int eval_one(nomad_ctx* c, const DevCsc& d, double* sse) {
+// SSE of one rating matrix: one FP64 partial per wave of 512 ratings, added
+// on the host in wave order.  The partials go to the context's own buffer
+// (allocated on first need) or, with `fixed`, to the library's scratch array.
+int eval_one(nomad_ctx* c, const DevCsc& d, double* sse, bool fixed) {
     *sse = 0.0;
     if (d.nnz <= 0) return NOMAD_OK;
     const int64_t chunk = 512;
     const int64_t waves = (d.nnz + chunk - 1) / chunk;
-    if (waves > c->partial_cap) {
-        dev_free(c->d_partial);
-        int rc = dev_alloc(&c->d_partial, size_t(waves + 4));
-        if (rc) return rc;
-        c->partial_cap = waves + 4;
+    double* partial = nullptr;
+    if (fixed) {
+        // the launch covers whole blocks of 4 waves, each writing its partial
+        if (waves + 4 > kScratchPartials)
+            return fail(NOMAD_EINVAL, "nomad_eval_noalloc: " + std::to_string(d.nnz) +
+                                          " ratings exceed the fixed scratch (" +
+                                          std::to_string((kScratchPartials - 4) * chunk) + " at most)");
+        partial = scratch_on(c->p.device);
+        if (!partial) return fail(NOMAD_ESTATE, "nomad_eval_noalloc: scratch address not resolved");
+    } else {
+        if (waves > c->partial_cap) {
+            dev_free(c->d_partial);
+            int rc = dev_alloc(&c->d_partial, size_t(waves + 4));
+            if (rc) return rc;
+            c->partial_cap = waves + 4;
+        }
+        partial = c->d_partial;
     }
     SseArgs a{};
     a.m = c->p.m;
@@ -1137,12 +1185,12 @@ int eval_one(nomad_ctx* c, const DevCsc& d, double* sse) {
     a.val = d.val;
     a.nnz = d.nnz;
     a.chunk = chunk;
-    a.partial = c->d_partial;
+    a.partial = partial;
     hipError_t e = dispatch_storage(c->p.storage,
                                     [&](auto t) { return launch_sse<decltype(t)::value>(a, waves, c->stream); });
     if (e != hipSuccess) return fail(NOMAD_EHIP, std::string("sse launch: ") + hipGetErrorString(e));
     std::vector<double> part(static_cast<size_t>(waves));
-    NOMAD_HIP_TRY(hipMemcpyAsync(part.data(), c->d_partial, size_t(waves) * 8, hipMemcpyDeviceToHost, c->stream));
+    NOMAD_HIP_TRY(hipMemcpyAsync(part.data(), partial, size_t(waves) * 8, hipMemcpyDeviceToHost, c->stream));
     NOMAD_HIP_TRY(hipStreamSynchronize(c->stream));
     double s = 0.0;
     for (double v : part) s += v;
@@ -1198,6 +1246,8 @@ int nomad_create(const nomad_params* p, nomad_ctx** out) {
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
         return bail(fail(NOMAD_EHIP, "nomad_create: stream/event creation failed"));
+    // the device address of nomad_eval_noalloc's scratch array, once per device
+    if (int rs = resolve_scratch(p->device)) return bail(rs);
 
     int P = p->nworkers;
     if (p->mode == NOMAD_MODE_ASYNC) {
@@ -1403,18 +1453,29 @@ int nomad_train_batch(nomad_ctx* const* ctxs, int32_t count, int64_t epochs, int
     return train_batch(ctxs, count, epochs, waves, stats);
 }
 
-int nomad_eval(nomad_ctx* c, nomad_eval_stats* st) {
-    if (!c || !st) return fail(NOMAD_EINVAL, "nomad_eval: null argument");
+namespace {
+int eval_both(nomad_ctx* c, nomad_eval_stats* st, bool fixed) {
     NOMAD_HIP_TRY(hipSetDevice(c->p.device));
     std::memset(st, 0, sizeof(*st));
     auto t0 = std::chrono::steady_clock::now();
     int rc;
-    if ((rc = eval_one(c, c->tr, &st->train_sse))) return rc;
-    if ((rc = eval_one(c, c->te, &st->test_sse))) return rc;
+    if ((rc = eval_one(c, c->tr, &st->train_sse, fixed))) return rc;
+    if ((rc = eval_one(c, c->te, &st->test_sse, fixed))) return rc;
     st->train_count = std::max<int64_t>(c->tr.nnz, 0);
     st->test_count = std::max<int64_t>(c->te.nnz, 0);
     st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return NOMAD_OK;
+}
+}  // namespace
+
+int nomad_eval(nomad_ctx* c, nomad_eval_stats* st) {
+    if (!c || !st) return fail(NOMAD_EINVAL, "nomad_eval: null argument");
+    return eval_both(c, st, false);
+}
+
+int nomad_eval_noalloc(nomad_ctx* c, nomad_eval_stats* st) {
+    if (!c || !st) return fail(NOMAD_EINVAL, "nomad_eval_noalloc: null argument");
+    return eval_both(c, st, true);
 }
 
 }  // extern "C"

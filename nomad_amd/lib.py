@@ -109,6 +109,7 @@ HIP_SYMBOLS = {
     "nomad_get_factors": (C.c_int, [_P, _f64p, _f64p]),
     "nomad_train": (C.c_int, [_P, C.c_double, C.c_int64, C.POINTER(TrainStats)]),
     "nomad_eval": (C.c_int, [_P, C.POINTER(EvalStats)]),
+    "nomad_eval_noalloc": (C.c_int, [_P, C.POINTER(EvalStats)]),
     "nomad_get_nworkers": (C.c_int, [_P, _i32p]),
     "nomad_row_blocks": (C.c_int, [_i32p, C.c_int64, C.c_int32, C.c_int32, _f64p, _i32p]),
     "nomad_get_counters": (C.c_int, [_P, _i64p, C.c_int32]),
@@ -496,6 +497,13 @@ class Context:
     def eval(self) -> EvalStats:
         st = EvalStats()
         self._check(self._lib.nomad_eval(self._h, C.byref(st)))
+        return st
+
+    def eval_noalloc(self) -> EvalStats:
+        """nomad_eval_noalloc: the same sums with no device allocation (the
+        library's fixed scratch array holds the partials)."""
+        st = EvalStats()
+        self._check(self._lib.nomad_eval_noalloc(self._h, C.byref(st)))
         return st
 
 

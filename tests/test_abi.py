@@ -54,3 +54,7 @@ def test_argument_errors_before_any_device_call():
     best = lib.TuneTrial()
     assert L.nomad_autotune(None, None, None, None, 0, 0.1, ctypes.byref(best), None, 0, None) == -1
     assert b"nomad_autotune" in L.nomad_last_error()
+    assert L.nomad_eval_noalloc(None, None) == -1
+    assert b"nomad_eval_noalloc" in L.nomad_last_error()
+    assert L.nomad_get_route_state(None, None, None, None, None, None, None, None, None) == -1
+    assert L.nomad_set_visit_counts(None, None, 0) == -1
