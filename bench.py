@@ -80,8 +80,18 @@ def interp_hit(rmse0, times, curve, target):
     return None
 
 
+def grad_norm(obj, ctx, reg):
+    """Stationarity of the current iterate: the objective L = sum (w.h - r)^2 +
+    reg (sum deg(i) |w_i|^2 + sum deg(j) |h_j|^2) and the Frobenius norm of its
+    full gradient, evaluated in FP64 by the library's objective kernels on the
+    context's factors (outside the timed region)."""
+    W, H = ctx.get_factors()
+    L, gW, gH = obj.eval_host(W, H, reg)
+    return float(L), float(np.sqrt(np.sum(gW * gW) + np.sum(gH * gH)))
+
+
 def run_path(lib, tr, te, W0, H0, *, storage, rounding, k, hyp, nworkers, warmup, steps, segment, target=None,
-             log_name=None):
+             log_name=None, obj=None):
     name = log_name or storage
     if warmup > 0:
         ctx = make_ctx(lib, tr, te, W0, H0, storage=storage, rounding=rounding, k=k, hyp=hyp, nworkers=nworkers)
@@ -89,13 +99,17 @@ def run_path(lib, tr, te, W0, H0, *, storage, rounding, k, hyp, nworkers, warmup
             ctx.train(seconds=segment)
         ctx.close()
     ctx = make_ctx(lib, tr, te, W0, H0, storage=storage, rounding=rounding, k=k, hyp=hyp, nworkers=nworkers)
-    curve, times = [], []
+    curve, times, grads, losses = [], [], [], []
     upd = 0
     dev_s = wall_s = alg = 0.0
-    visits = 0
+    visits = hops = 0
     t_hit = None
     ev0 = ctx.eval()
     rmse0 = float(np.sqrt(ev0.test_sse / max(ev0.test_count, 1)))  # before training
+    if obj is not None:
+        L, g = grad_norm(obj, ctx, hyp["reg"])
+        losses.append(L)
+        grads.append(g)
     for step in range(steps):
         t0 = time.perf_counter()
         st = ctx.train(seconds=segment)
@@ -104,19 +118,27 @@ def run_path(lib, tr, te, W0, H0, *, storage, rounding, k, hyp, nworkers, warmup
         upd += st.updates
         alg += st.alg_bytes
         visits += st.visits
+        hops += ctx.counters()["claimed"]
         ev = ctx.eval()
         rmse = float(np.sqrt(ev.test_sse / max(ev.test_count, 1)))
         curve.append(round(rmse, 6))
         times.append(round(dev_s, 4))
         if target is not None and t_hit is None and rmse <= target:
             t_hit = dev_s
+        gtxt = ""
+        if obj is not None:
+            L, g = grad_norm(obj, ctx, hyp["reg"])
+            losses.append(L)
+            grads.append(g)
+            gtxt = f" L={L:.6g} |grad L|={g:.6g}"
         log(f"{name}: segment {step + 1} updates={st.updates} dev={st.train_seconds * 1e3:.1f} ms "
-            f"epochs(min over columns)={st.epochs_done} test RMSE={rmse:.6f}")
+            f"epochs(min over columns)={st.epochs_done} test RMSE={rmse:.6f}{gtxt}")
     nw = ctx.nworkers
+    rs = ctx.route_state()
     ctx.close()
-    return dict(updates=upd, dev_s=dev_s, wall_s=wall_s, alg_bytes=alg, visits=visits, curve=curve, times=times,
-                rmse0=rmse0,
-                t_hit=t_hit, nworkers=nw)
+    return dict(updates=upd, dev_s=dev_s, wall_s=wall_s, alg_bytes=alg, visits=visits, hops=hops, curve=curve,
+                times=times, rmse0=rmse0, t_hit=t_hit, nworkers=nw, grads=grads, losses=losses,
+                visit_count_range=[int(rs["count"].min()), int(rs["count"].max())])
 
 
 def profiled_traffic(storage: str) -> float | None:
@@ -153,15 +175,50 @@ def copy_calibrate(nbytes: int) -> None:
     log(f"copy calibration: {nbytes} bytes device to device")
 
 
-def cpu_baseline(tr, W0, H0, k, hyp, seconds):
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(tr, te, W0, H0, k, hyp, seconds, segment, target):
+    """The oracle's restatement of the reference nomad_float async loop
+    (nb_main.cpp:415-578, per-thread visit counts, set_perm after every
+    timeout) through the same timeout schedule as the GPU paths, with longer
+    segments: train a segment, stop, evaluate the test RMSE (excluded from
+    the training time, nb_main.cpp:265-266), until `seconds` of training or
+    the target RMSE is reached."""
     from oracle import oracle as O
 
-    threads = min(16, os.cpu_count() or 1)
-    Wc, Hc, st = O.sgd_async(tr, W0, H0, storage=O.FP32, T=threads, seed=12345, seconds=seconds, **hyp)
-    return dict(value=st.updates / st.seconds, unit="updates/s", cores=threads, kind="port",
+    # the host cores this job may use: the box gives one GPU's job 16
+    threads = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16)))
+    run = O.AsyncRun(tr, W0, H0, storage=O.FP32, T=threads, seed=12345, **hyp)
+    train_s, upd = 0.0, 0
+    curve, times = [], []
+    t_hit = None
+    while train_s < seconds - 1e-9:
+        st = run.segment(seconds=min(segment, seconds - train_s))
+        train_s += st.seconds
+        upd += st.updates
+        W, H = run.factors()
+        rmse = float(np.sqrt(O.sse(te, W, H, storage=O.FP32, arith=O.ARITH_REF) / te.nnz))
+        curve.append(round(rmse, 6))
+        times.append(round(train_s, 3))
+        log(f"cpu: {train_s:.1f} s {upd} updates test RMSE {rmse:.6f}")
+        if target is not None and rmse <= target:
+            t_hit = train_s
+            break
+    run.close()
+    return dict(value=upd / train_s, unit="updates/s", cores=threads, kind="port", cpu_model=cpu_model(),
                 sample=f"CPU restatement of the reference nomad_float async loop (nb_main.cpp:415-578), "
-                       f"{threads} threads, {st.seconds:.1f} s timeout on the full training set, "
-                       f"{st.updates} updates")
+                       f"{threads} threads, {segment:g} s timeout segments on the full training set until "
+                       f"{seconds:g} s of training or the target RMSE, {upd} updates",
+                train_seconds=times, test_rmse_curve=curve, time_to_target_s=t_hit,
+                rmse0=None)
 
 
 def sweep(a, lib) -> dict:
@@ -353,6 +410,16 @@ def yahoo_workload(a, lib) -> dict:
                        "train_seconds": st.train_seconds, "epochs": int(st.epochs_done)}
         log(f"yahoo: gpu {name}: {a.epochs} epochs in {st.train_seconds:.2f} s "
             f"({st.updates / st.train_seconds / 1e9:.3f} G updates/s), test RMSE {rmse:.5f}")
+        # timed segments (the benchmark's mode): columns circulate for a
+        # fixed device time, so no launch waits for the heaviest column
+        upd = dev = 0.0
+        for _ in range(a.yahoo_segments):
+            s2 = ctx.train(seconds=a.segment)
+            upd += s2.updates
+            dev += s2.train_seconds
+        paths[name]["timed_updates_per_s"] = upd / dev
+        paths[name]["timed_segments"] = a.yahoo_segments
+        log(f"yahoo: gpu {name}: {a.yahoo_segments} timed {a.segment} s segments: {upd / dev / 1e9:.3f} G updates/s")
         ctx.close()
     # cpu_baseline leg: the oracle's FP64 async restatement on the host, the
     # same epochs; its test RMSE is the FP64 reference of the gaps
@@ -386,11 +453,14 @@ def main() -> int:
     ap.add_argument("--segment", type=float, default=0.1, help="device seconds per segment")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--nworkers", type=int, default=2048)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=24.0, help="CPU baseline: training seconds at most")
+    ap.add_argument("--cpu-segment", type=float, default=3.0, help="CPU baseline: seconds per timeout segment")
+    ap.add_argument("--no-grad", action="store_true", help="skip the per-segment gradient norms")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp32", "bf16", "bf16_sr"], default=None)
     ap.add_argument("--workload", choices=["netflix", "sweep", "ccd", "objective", "yahoo"], default="netflix")
     ap.add_argument("--epochs", type=int, default=3, help="yahoo: epochs per path")
+    ap.add_argument("--yahoo-segments", type=int, default=5, help="yahoo: timed segments per path after the epochs")
     ap.add_argument("--ccd-rank", type=int, default=100)
     ap.add_argument("--ccd-inner", type=int, default=3)
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
@@ -422,7 +492,8 @@ def main() -> int:
     tr, te = lib.synth(spec)
     log(f"generated {tr.m} x {tr.n}, train {tr.nnz}, test {te.nnz} in {time.perf_counter() - t0:.1f} s")
     W0, H0 = lib.init_factors(tr.m, tr.n, k, 4, 12345)
-    common = dict(k=k, hyp=hyp, nworkers=a.nworkers, warmup=a.warmup, steps=a.steps, segment=a.segment)
+    obj = None if a.no_grad else lib.Objective(tr, k, storage="fp64")
+    common = dict(k=k, hyp=hyp, nworkers=a.nworkers, warmup=a.warmup, steps=a.steps, segment=a.segment, obj=obj)
 
     res = {}
     target = None
@@ -439,6 +510,8 @@ def main() -> int:
 
     for r in res.values():
         r["t_hit_interp"] = interp_hit(r["rmse0"], r["times"], r["curve"], target)
+    if obj is not None:
+        obj.close()
 
     head_name = next(n for n in ("bf16", "bf16_sr", "fp32") if n in res)
     head = res[head_name]
@@ -446,6 +519,11 @@ def main() -> int:
     achieved = head["alg_bytes"] / head["dev_s"] / 1e9
     tpu = profiled_traffic(head_name)
     traffic = tpu * head["updates"] / a.steps if tpu is not None else None
+    # the h term counted once per (column, workgroup) hop instead of once per
+    # (column, wave) visit: inside a workgroup h moves through LDS
+    esz = 4 if head_name == "fp32" else 2
+    alg_hop = head["alg_bytes"] - (head["visits"] - head["hops"]) * 2.0 * k * esz
+    achieved_hop = alg_hop / head["dev_s"] / 1e9
     out = {
         "metric": "updates/s + time-to-target test RMSE, Netflix-shape k=100, BF16 vs FP32, 1 GPU",
         "value": value,
@@ -469,6 +547,9 @@ def main() -> int:
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "frac_h_per_hop": achieved_hop / HBM_PEAK_GBS,
+            "frac_measured_traffic": (traffic / (head["dev_s"] / a.steps) / 1e9 / HBM_PEAK_GBS
+                                      if traffic is not None else None),
             "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per update (copy-calibrated), "
                               "profiles/sgd_async_kernel_netflix_k100_pmc.md, x updates per launch",
             "kernel": "sgd_async_kernel<" + {"fp32": "fp32", "bf16": "bf16, nearest",
@@ -487,13 +568,18 @@ def main() -> int:
                 "test_rmse_curve": r["curve"],
                 "time_to_target_s": r["t_hit"],
                 "time_to_target_interp_s": r["t_hit_interp"],
+                "hops_per_segment": r["hops"] / a.steps,
+                "objective_curve": [round(x, 6) for x in r["losses"]],
+                "grad_norm_curve": [round(x, 6) for x in r["grads"]],
+                "visit_count_range": r["visit_count_range"],
             } for name, r in res.items()
         },
         "target_test_rmse": target,
     }
     if not a.skip_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(tr, W0, H0, k, hyp, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(tr, te, W0, H0, k, hyp, a.cpu_seconds, a.cpu_segment, target)
+        out["cpu_baseline"]["rmse0"] = head["rmse0"]
     print(json.dumps(out), flush=True)
     return 0
 

@@ -10,6 +10,8 @@ import time
 import numpy as np
 import pytest
 
+from oracle import oracle as O
+
 pytestmark = pytest.mark.gpu
 
 STRATA = [3, 8, 16, 5, 1, 32, 12, 7]
@@ -164,4 +166,68 @@ def test_autotune_batch_picks_a_valid_common_strata_count(gpu):
     st = gpu.train_batch(cs, 1)
     assert all(s.updates == p[0].nnz for s, p in zip(st, probs))
     for c in cs:
+        c.close()
+
+
+def _sr_seed(seed):
+    return ((seed ^ (seed >> 32)) & 0xFFFFFFFF) ^ 0xA5A5A5A5  # the library's stochastic-rounding key
+
+
+@pytest.mark.parametrize("storage", ["bf16", "fp32"])
+def test_batch_k64_k128_B256_vs_oracle(gpu, storage):
+    """Problems of the sweep with k = 64 and k = 128 at 1/100 of their sizes,
+    256 strata each (capped at the rows), trained in one batched launch, are
+    bit-identical to the oracle's replay of the same stratified schedule in
+    the kernels' arithmetic (fp32, and bf16 with stochastic rounding)."""
+    found = {}
+    p = 0
+    while len(found) < 2 and p < 256:
+        spec, k, reg = gpu.preset_batch(p, 0.01)
+        if k in (64, 128) and k not in found:
+            found[k] = (p, spec, reg)
+        p += 1
+    assert set(found) == {64, 128}
+    ctxs, probs = [], []
+    for i, (k, (p, spec, reg)) in enumerate(sorted(found.items())):
+        tr, te = gpu.synth(spec)
+        B = min(256, tr.m)
+        c = _ctx(gpu, tr, te, k, reg, B, storage, seed=i)
+        ctxs.append(c)
+        probs.append((tr, k, reg, B, i))
+    gpu.train_batch(ctxs, 2)
+    for c, (tr, k, reg, B, seed) in zip(ctxs, probs):
+        W0, H0 = gpu.init_factors(tr.m, tr.n, k, 2, 1000 + seed)
+        ost = O.BF16 if storage == "bf16" else O.FP32
+        rnd = O.ROUND_STOCHASTIC if storage == "bf16" else O.ROUND_NEAREST
+        Wo, Ho = O.sgd_strat(tr, W0, H0, storage=ost, arith=O.ARITH_GPU, rounding=rnd, sr_seed=_sr_seed(seed), B=B,
+                             epochs=2, lrate=0.002, drate=0.05, reg=reg)
+        Wg, Hg = c.get_factors()
+        print(f"[batch vs oracle {storage}] m={tr.m} n={tr.n} nnz={tr.nnz} k={k} B={B}", flush=True)
+        assert np.array_equal(Wg, Wo) and np.array_equal(Hg, Ho), (k, np.abs(Wg - Wo).max())
+        c.close()
+
+
+def test_batch_sweep_full_size_autotuned(gpu):
+    """The full batched sweep: 32 problems at full size (about 161 M ratings),
+    the batch autotuner's common strata count, one batched epoch bit-identical
+    to the same epoch of every problem run on its own."""
+    probs = _problems(gpu, 32, 1.0)
+    total = sum(p[0].nnz for p in probs)
+    print(f"[sweep full] 32 problems, {total} training ratings", flush=True)
+    strata, rate = gpu.autotune_batch([(tr, k, reg) for tr, te, k, reg in probs])
+    print(f"[sweep full] autotuned strata {sorted(set(strata))}, {rate / 1e9:.3f} G updates/s", flush=True)
+    batch = [_ctx(gpu, tr, te, k, reg, B, "bf16", seed=i) for i, ((tr, te, k, reg), B) in enumerate(zip(probs, strata))]
+    st = gpu.train_batch(batch, 1)
+    print(f"[sweep full] batched epoch {st[0].train_seconds * 1e3:.1f} ms", flush=True)
+    t_seq = 0.0
+    for i, ((tr, te, k, reg), B) in enumerate(zip(probs, strata)):
+        a = _ctx(gpu, tr, te, k, reg, B, "bf16", seed=i)
+        t_seq += a.train(epochs=1).train_seconds
+        Wa, Ha = a.get_factors()
+        Wb, Hb = batch[i].get_factors()
+        assert st[i].updates == tr.nnz
+        assert np.array_equal(Wa, Wb) and np.array_equal(Ha, Hb), f"problem {i} differs"
+        a.close()
+    print(f"[sweep full] one problem after another {t_seq * 1e3:.1f} ms", flush=True)
+    for c in batch:
         c.close()

@@ -126,3 +126,25 @@ def test_central_differences_config4_shape():
         fd = (part(i, xp) - part(i, xm)) / (2 * eps)
         worst = max(worst, abs(fd - G[i, d]) / (1e-6 + 1e-6 * abs(G[i, d])))
     assert worst <= 1.0, worst
+
+
+def test_objective_fp32_config4_shape():
+    """The FP32 leg at the full 100,000 x 20,000, 10 M-rating, k = 64 shape:
+    FP32 arithmetic with FP64 sums stays within 1e-5 (objective) and 1e-4
+    (gradient, relative Frobenius norm) of the FP64 evaluation of the same
+    factors, and the FP64 value equals the oracle's."""
+    tr, _ = lib.synth(lib.preset(4, 1.0))
+    k, reg = 64, 0.05
+    print(f"[obj fp32] config-4 shape {tr.m}x{tr.n} nnz={tr.nnz} k={k}", flush=True)
+    W, H = lib.init_factors(tr.m, tr.n, k, 4, 12345)
+    W = W.astype(np.float32).astype(np.float64)  # values exact in both precisions
+    H = H.astype(np.float32).astype(np.float64)
+    L64, gW64, gH64 = lib.Objective(tr, k, "fp64").eval_host(W, H, reg)
+    L32, gW32, gH32 = lib.Objective(tr, k, "fp32").eval_host(W, H, reg)
+    Lo, _, _ = O.loss_grad(tr, W, H, reg, want_grad=False)
+    assert L64 == pytest.approx(Lo, rel=1e-12)
+    assert L32 == pytest.approx(L64, rel=1e-5)
+    num = np.sqrt(np.sum((gW32 - gW64) ** 2) + np.sum((gH32 - gH64) ** 2))
+    den = np.sqrt(np.sum(gW64 ** 2) + np.sum(gH64 ** 2))
+    print(f"[obj fp32] L rel {abs(L32 - L64) / L64:.3e}, gradient rel {num / den:.3e}", flush=True)
+    assert num <= 1e-4 * den
