@@ -15,6 +15,11 @@
  *       synthetic matrices with a planted low-rank truth; presets for the
  *       benchmark shapes (small uniform, Netflix, Yahoo! Music, gradient check,
  *       batched-sweep problems)
+ *   nomad_tools_convert_tsv
+ *       Scripts/convert.py: TSV rating files to train.dat / test.dat
+ *   nomad_tools_synthgen_conf / nomad_tools_synthgen
+ *       the reference's own synthetic generator (Code/nomad/synthgen-conf.cpp,
+ *       synthgen.cpp, gen.sh)
  *
  * Return codes: 0 on success, negative on error (message via
  * nomad_tools_last_error()).
@@ -72,6 +77,38 @@ int nomad_tools_preset_batch(int32_t p, double scale, nomad_synth_spec* out, int
  * nthreads is the reference's --nthreads (number of row parts). */
 int nomad_tools_init_factors(int32_t m, int32_t n, int32_t k, int32_t nthreads, uint64_t seed,
                              double* W, double* H);
+
+/* Scripts/convert.py:1-147: "user<TAB>item<TAB>rating" lines of train_path
+ * and test_path -> out_dir/train.dat and test.dat (class id 1211216, rows =
+ * users, duplicate (user, item) lines summed, columns ascending in a row, as
+ * scipy's csr_matrix builds them) plus out_dir/user_ids.txt and item_ids.txt
+ * (the ids in index order).  Ids are indexed after a shuffle with Python's
+ * random generator seeded `seed` (convert.py uses 12345), i.e. random.seed
+ * then random.shuffle as Python 2 performs it; the ids enter the shuffle in
+ * order of first appearance (train, then test), where convert.py starts from
+ * a Python 2 set's iteration order.  *nusers, *nitems (may be NULL) receive
+ * the matrix size. */
+int nomad_tools_convert_tsv(const char* train_path, const char* test_path, const char* out_dir, uint64_t seed,
+                            int64_t* nusers, int64_t* nitems);
+
+/* Code/nomad/synthgen-conf.cpp:74-122: the degree-frequency files of the
+ * reference's synthetic generator.  row_path gets "d<sep>f" for d = 0..ncol
+ * with f = int(N(20, 5)) redrawn until in [0, ncol); col_path gets "d<sep>f"
+ * for d = 0..nrow with f = int of a power law (xmin 1, exponent 2.2,
+ * Code/nomad/pl-dis.hpp) redrawn while above nrow; std::mt19937(seed). */
+int nomad_tools_synthgen_conf(int32_t nrow, int32_t ncol, int32_t seed, const char* row_path, const char* col_path,
+                              const char* sep);
+
+/* Code/nomad/synthgen.cpp:108-311 and gen.sh:28-55: out_dir/train.dat and
+ * test.dat (class id 1015) of an nrow x ncol matrix: column weights drawn
+ * from the column file, H (dim x ncol) and each row's factor ~ N(0, 1), row
+ * degree from the row file x 5/4 (at most ncol), 80 % of it training
+ * columns and the rest test columns, each drawn by weight without repeats
+ * within its split, value = row . column + 0.1 N(0, 1).  The same libstdc++
+ * distributions in the reference's order, std::mt19937(seed).  *train_nnz,
+ * *test_nnz (may be NULL) receive the counts. */
+int nomad_tools_synthgen(const char* row_path, const char* col_path, const char* out_dir, int32_t nrow,
+                         int32_t ncol, int32_t dim, int32_t seed, int64_t* train_nnz, int64_t* test_nnz);
 
 #ifdef __cplusplus
 }

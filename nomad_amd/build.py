@@ -8,6 +8,9 @@ Products (all git-ignored, shipped with the tree):
     nomad_amd/lib/libnomad_tools.so  .dat I/O, generators (include/nomad_tools.h)
     nomad_amd/bin/nomad_float        drop-in host driver, fp32 storage
     nomad_amd/bin/nomad_double       drop-in host driver, fp64 storage
+    nomad_amd/bin/nomad_convert, nomad_synthgen_conf, nomad_synthgen, nomad_gen
+                                     the reference's data tools (Scripts/convert.py,
+                                     synthgen-conf, synthgen, gen.sh)
 """
 from __future__ import annotations
 
@@ -42,7 +45,9 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", f"-I{CSRC}"]
 
 HIP_SOURCES = ["sgd_kernels.hip", "async_kernel.hip", "eval_kernels.hip", "ccd.hip", "objective.hip", "capi.hip",
                "autotune.hip"]
-TOOLS_SOURCES = ["datfile.cpp", "synth.cpp", "tools_capi.cpp"]
+TOOLS_SOURCES = ["datfile.cpp", "synth.cpp", "convert.cpp", "tools_capi.cpp"]
+# command-line front ends of libnomad_tools.so: one program, one name per reference tool
+TOOL_NAMES = ["nomad_convert", "nomad_synthgen_conf", "nomad_synthgen", "nomad_gen"]
 
 
 def _run(cmd: list[str]) -> None:
@@ -97,6 +102,15 @@ def build_bin() -> list[Path]:
         _run([CXX, *CXX_FLAGS, f"-DNOMAD_DEFAULT_STORAGE={storage}", "-pthread", str(main),
               "-L", str(LIBDIR), "-lnomad_hip", "-lnomad_tools",
               f"-Wl,-rpath,$ORIGIN/../lib", "-o", str(out)])
+        outs.append(out)
+    first = BINDIR / TOOL_NAMES[0]
+    _run([CXX, *CXX_FLAGS, str(CSRC / "tools_main.cpp"), "-L", str(LIBDIR), "-lnomad_tools",
+          f"-Wl,-rpath,$ORIGIN/../lib", "-o", str(first)])
+    outs.append(first)
+    for name in TOOL_NAMES[1:]:
+        out = BINDIR / name
+        out.write_bytes(first.read_bytes())
+        out.chmod(0o755)
         outs.append(out)
     return outs
 

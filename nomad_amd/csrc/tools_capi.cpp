@@ -5,6 +5,7 @@
 #include <random>
 #include <string>
 
+#include "convert.h"
 #include "datfile.h"
 #include "nomad_tools.h"
 #include "synth.h"
@@ -264,6 +265,44 @@ int nomad_tools_init_factors(int32_t m, int32_t n, int32_t k, int32_t nthreads, 
         std::uniform_real_distribution<double> dist(0, hi);
         for (int64_t e = 0; e < int64_t(n) * k; ++e) H[e] = dist(rng);
     }
+    return 0;
+}
+
+int nomad_tools_convert_tsv(const char* train_path, const char* test_path, const char* out_dir, uint64_t seed,
+                            int64_t* nusers, int64_t* nitems) {
+    if (!train_path || !test_path || !out_dir) return tfail("nomad_tools_convert_tsv: null path");
+    std::string err;
+    int64_t nu = 0, ni = 0;
+    try {
+        if (!nomad::convert_tsv(train_path, test_path, out_dir, seed, nu, ni, err)) return tfail(err);
+    } catch (const std::exception& e) {
+        return tfail(std::string("nomad_tools_convert_tsv: ") + e.what());
+    }
+    if (nusers) *nusers = nu;
+    if (nitems) *nitems = ni;
+    return 0;
+}
+
+int nomad_tools_synthgen_conf(int32_t nrow, int32_t ncol, int32_t seed, const char* row_path, const char* col_path,
+                              const char* sep) {
+    if (!row_path || !col_path) return tfail("nomad_tools_synthgen_conf: null path");
+    std::string err;
+    if (!nomad::synthgen_conf(nrow, ncol, seed, row_path, col_path, sep ? sep : ",", err)) return tfail(err);
+    return 0;
+}
+
+int nomad_tools_synthgen(const char* row_path, const char* col_path, const char* out_dir, int32_t nrow,
+                         int32_t ncol, int32_t dim, int32_t seed, int64_t* train_nnz, int64_t* test_nnz) {
+    if (!row_path || !col_path || !out_dir) return tfail("nomad_tools_synthgen: null path");
+    std::string err;
+    int64_t a = 0, b = 0;
+    try {
+        if (!nomad::synthgen(row_path, col_path, out_dir, nrow, ncol, dim, seed, a, b, err)) return tfail(err);
+    } catch (const std::exception& e) {
+        return tfail(std::string("nomad_tools_synthgen: ") + e.what());
+    }
+    if (train_nnz) *train_nnz = a;
+    if (test_nnz) *test_nnz = b;
     return 0;
 }
 

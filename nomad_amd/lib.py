@@ -143,6 +143,10 @@ TOOLS_SYMBOLS = {
     "nomad_tools_preset_batch": (C.c_int, [C.c_int32, C.c_double, C.POINTER(SynthSpec), _i32p, _f64p]),
     "nomad_tools_init_factors": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
                                            _f64p, _f64p]),
+    "nomad_tools_convert_tsv": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint64, _i64p, _i64p]),
+    "nomad_tools_synthgen_conf": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_char_p, C.c_char_p, C.c_char_p]),
+    "nomad_tools_synthgen": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int32, C.c_int32, C.c_int32,
+                                       C.c_int32, _i64p, _i64p]),
 }
 
 _libs: dict[str, C.CDLL] = {}
@@ -245,6 +249,26 @@ def write_dat(path: str | os.PathLike, a: Csc, long_format: bool = False) -> Non
         _tools_check(L.nomad_tools_write_dat(str(path).encode(), h, int(long_format)))
     finally:
         L.nomad_tools_matrix_free(h)
+
+
+def convert_tsv(train_path, test_path, out_dir, seed: int = 12345) -> tuple[int, int]:
+    """Scripts/convert.py's conversion (nomad_tools_convert_tsv): returns (users, items)."""
+    nu, ni = C.c_int64(), C.c_int64()
+    _tools_check(tools().nomad_tools_convert_tsv(str(train_path).encode(), str(test_path).encode(),
+                                                 str(out_dir).encode(), seed, C.byref(nu), C.byref(ni)))
+    return nu.value, ni.value
+
+
+def synthgen_conf(nrow: int, ncol: int, seed: int, row_path, col_path, sep: str = ",") -> None:
+    _tools_check(tools().nomad_tools_synthgen_conf(nrow, ncol, seed, str(row_path).encode(), str(col_path).encode(),
+                                                   sep.encode()))
+
+
+def synthgen(row_path, col_path, out_dir, nrow: int, ncol: int, dim: int, seed: int) -> tuple[int, int]:
+    a, b = C.c_int64(), C.c_int64()
+    _tools_check(tools().nomad_tools_synthgen(str(row_path).encode(), str(col_path).encode(), str(out_dir).encode(),
+                                              nrow, ncol, dim, seed, C.byref(a), C.byref(b)))
+    return a.value, b.value
 
 
 def preset(config: int, scale: float = 1.0) -> SynthSpec:

@@ -159,3 +159,267 @@ def test_global_csc_helper_merges_reference_parts(tmp_path, nparts):
     np.testing.assert_array_equal(colptr, b.colptr)
     np.testing.assert_array_equal(rowidx, b.rowidx)
     np.testing.assert_array_equal(val, b.val)
+
+
+# ---- the reference's data tools: Scripts/convert.py, synthgen-conf, synthgen ----
+class MT32:
+    """std::mt19937 (Matsumoto & Nishimura, 32-bit), seeded like the standard
+    engine's constructor."""
+
+    def __init__(self, seed):
+        self.mt = [seed & 0xFFFFFFFF]
+        for i in range(1, 624):
+            x = self.mt[-1]
+            self.mt.append((1812433253 * (x ^ (x >> 30)) + i) & 0xFFFFFFFF)
+        self.i = 624
+
+    def __call__(self):
+        if self.i >= 624:
+            mt = self.mt
+            for k in range(624):
+                y = (mt[k] & 0x80000000) | (mt[(k + 1) % 624] & 0x7FFFFFFF)
+                mt[k] = mt[(k + 397) % 624] ^ (y >> 1) ^ (0x9908B0DF if y & 1 else 0)
+            self.i = 0
+        y = self.mt[self.i]
+        self.i += 1
+        y ^= y >> 11
+        y ^= (y << 7) & 0x9D2C5680
+        y ^= (y << 15) & 0xEFC60000
+        y ^= y >> 18
+        return y & 0xFFFFFFFF
+
+
+def test_mt32_standard_value():
+    g = MT32(5489)
+    for _ in range(9999):
+        g()
+    assert g() == 4123659995  # [rand.predef] mt19937, 10000th output
+
+
+def _canonical(g):
+    # libstdc++ generate_canonical<double, 53> on a 32-bit engine: two draws
+    s = float(g()) + float(g()) * 4294967296.0
+    u = s / 18446744073709551616.0
+    return u if u < 1.0 else np.nextafter(1.0, 0.0)
+
+
+class Normal:
+    """libstdc++ normal_distribution: Marsaglia's polar method, the second
+    value of each pair kept for the next call."""
+
+    def __init__(self, mean, sd):
+        self.mean, self.sd, self.saved = mean, sd, None
+
+    def __call__(self, g):
+        import math
+
+        if self.saved is not None:
+            r, self.saved = self.saved, None
+        else:
+            while True:
+                x = 2.0 * _canonical(g) - 1.0
+                y = 2.0 * _canonical(g) - 1.0
+                r2 = x * x + y * y
+                if not (r2 > 1.0 or r2 == 0.0):
+                    break
+            mult = math.sqrt(-2 * math.log(r2) / r2)
+            self.saved = x * mult
+            r = y * mult
+        return r * self.sd + self.mean
+
+
+class Discrete:
+    """libstdc++ discrete_distribution: weights over their sum, cumulative
+    sums (last set to 1), the first index whose cumulative sum reaches one
+    canonical draw."""
+
+    def __init__(self, w):
+        import itertools
+
+        s = 0.0
+        for x in w:
+            s += float(x)
+        self.cp = list(itertools.accumulate(float(x) / s for x in w))
+        self.cp[-1] = 1.0
+
+    def __call__(self, g):
+        import bisect
+
+        return bisect.bisect_left(self.cp, _canonical(g))
+
+
+def _restated_conf(nrow, ncol, seed):
+    import math
+
+    g = MT32(seed)
+    nd = Normal(20.0, 5.0)
+    rows = []
+    for i in range(ncol + 1):
+        v = int(nd(g))
+        while v < 0 or v >= ncol:
+            v = int(nd(g))
+        rows.append((i, v))
+    cols = []
+    ex = 1.0 / (1 - 2.2)
+    for i in range(nrow + 1):
+        v = int(1.0 * math.pow(1.0 - _canonical(g), ex))
+        while v > nrow:
+            v = int(1.0 * math.pow(1.0 - _canonical(g), ex))
+        cols.append((i, v))
+    return rows, cols
+
+
+def _restated_gen(rows, cols, nrow, ncol, dim, seed):
+    g = MT32(seed)
+    col_deg_dist = Discrete([f for _, f in cols])
+    row_deg_dist = Discrete([f for _, f in rows])
+    nd = Normal(0.0, 1.0)
+    weights = [cols[col_deg_dist(g)][0] for _ in range(ncol)]
+    col_dist = Discrete(weights)
+    H = [nd(g) for _ in range(dim * ncol)]
+    out = {"train": ([], [], []), "test": ([], [], [])}
+    for _ in range(nrow):
+        u = [nd(g) for _ in range(dim)]
+        deg = min(rows[row_deg_dist(g)][0] * 5 // 4, ncol)
+        ntr = int(deg * 0.8)
+        tr, te = set(), set()
+        for j in range(deg):
+            while True:
+                c = col_dist(g)
+                s = tr if j < ntr else te
+                if c not in s:
+                    s.add(c)
+                    break
+        for name, s in (("train", tr), ("test", te)):
+            sizes, ind, val = out[name]
+            for c in sorted(s):
+                t = 0.0
+                for q in range(dim):
+                    t += u[q] * H[dim * c + q]
+                t += nd(g) * 0.1
+                ind.append(c)
+                val.append(t)
+            sizes.append(len(s))
+    files = {}
+    for name, (sizes, ind, val) in out.items():
+        files[name] = (struct.pack("<iiiq", 1015, nrow, ncol, sum(sizes)) + struct.pack(f"<{len(sizes)}I", *sizes)
+                       + struct.pack(f"<{len(ind)}i", *ind) + struct.pack(f"<{len(val)}d", *val))
+    return files
+
+
+@pytest.mark.parametrize("nrow,ncol,seed", [(30, 12, 12345), (7, 40, 3)])
+def test_synthgen_conf_matches_restatement(tmp_path, nrow, ncol, seed):
+    """nomad_synthgen_conf's degree files equal an independent restatement of
+    synthgen-conf.cpp:74-122 on libstdc++'s mt19937, polar normal and
+    canonical draws, value for value."""
+    rf, cf = tmp_path / "row.txt", tmp_path / "col.txt"
+    lib.synthgen_conf(nrow, ncol, seed, rf, cf)
+    rows, cols = _restated_conf(nrow, ncol, seed)
+    assert rf.read_text() == "".join(f"{i},{v}\n" for i, v in rows)
+    assert cf.read_text() == "".join(f"{i},{v}\n" for i, v in cols)
+
+
+@pytest.mark.parametrize("nrow,ncol,dim,seed", [(40, 12, 3, 12345), (25, 30, 2, 9)])
+def test_synthgen_matches_restatement(tmp_path, nrow, ncol, dim, seed):
+    """nomad_synthgen's train.dat / test.dat (gen.sh's concatenation of
+    header, row sizes, column indexes and values) equal, byte for byte, an
+    independent restatement of synthgen.cpp:182-311 on libstdc++'s
+    discrete and normal distributions."""
+    rf, cf = tmp_path / "row.txt", tmp_path / "col.txt"
+    lib.synthgen_conf(nrow, ncol, seed, rf, cf)
+    ntr, nte = lib.synthgen(rf, cf, tmp_path, nrow, ncol, dim, seed)
+    rows, cols = _restated_conf(nrow, ncol, seed)
+    want = _restated_gen(rows, cols, nrow, ncol, dim, seed)
+    assert (tmp_path / "train.dat").read_bytes() == want["train"]
+    assert (tmp_path / "test.dat").read_bytes() == want["test"]
+    tr = lib.read_dat(tmp_path / "train.dat")
+    assert (tr.m, tr.n, tr.nnz) == (nrow, ncol, ntr)
+
+
+def test_synthgen_moments(tmp_path):
+    """At 2,000 x 400 with dim 8: row-degree frequencies ~ int(N(20, 5))
+    (mean 19.5 after truncation, sd 5), column-degree frequencies follow the
+    2.2 power law (P(>= 2) = 2^-1.2, P(>= 10) = 10^-1.2), each row's split is
+    80/20, and the values have variance dim + 0.01 (unit-normal factors plus
+    0.1 N(0, 1) noise)."""
+    nrow, ncol, dim = 2000, 400, 8
+    rf, cf = tmp_path / "row.txt", tmp_path / "col.txt"
+    lib.synthgen_conf(nrow, ncol, 12345, rf, cf)
+    rfreq = np.array([int(l.split(",")[1]) for l in rf.read_text().splitlines()])
+    cfreq = np.array([int(l.split(",")[1]) for l in cf.read_text().splitlines()])
+    assert abs(rfreq.mean() - 19.5) < 0.6 and abs(rfreq.std() - 5.0) < 0.5
+    for x in (2, 10):
+        p = (cfreq >= x).mean()
+        assert abs(p - x ** -1.2) < 4 * np.sqrt(x ** -1.2 * (1 - x ** -1.2) / cfreq.size), (x, p)
+    lib.synthgen(rf, cf, tmp_path, nrow, ncol, dim, 12345)
+    raw = (tmp_path / "train.dat").read_bytes()
+    nnz = struct.unpack_from("<q", raw, 12)[0]
+    sizes = np.frombuffer(raw, np.uint32, nrow, 20)
+    raw_te = (tmp_path / "test.dat").read_bytes()
+    sizes_te = np.frombuffer(raw_te, np.uint32, nrow, 20)
+    deg = sizes + sizes_te
+    assert (sizes == (deg * 0.8).astype(np.int64)).all()
+    vals = np.frombuffer(raw, np.float64, nnz, 20 + 4 * nrow + 4 * nnz)
+    assert abs(vals.var() / (dim + 0.01) - 1) < 0.1 and abs(vals.mean()) < 0.1
+
+
+def test_convert_tsv_matches_convert_py_procedure(tmp_path):
+    """nomad_convert: ids in order of first appearance, shuffled by Python's
+    own random.seed(12345) + the Python 2 shuffle (j = int(random() * (i+1))),
+    the files written as scipy's csr_matrix would (duplicates summed, columns
+    ascending), plus the id lists."""
+    import random
+
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(0)
+    users = [f"u{v}" for v in rng.integers(1000, 2000, 60)]
+    items = [f"i{v}" for v in rng.integers(10, 90, 60)]
+    lines_tr = [f"{users[q]}\t{items[q]}\t{q % 5 + 1}\t{q}\n" for q in range(50)]
+    lines_tr.append(f"{users[3]}\t{items[3]}\t2.5\t0\n")  # a duplicate (user, item): summed
+    lines_te = [f"{users[q]}\t{items[q]}\t{(q % 3) + 0.5}\n" for q in range(45, 60)]
+    (tmp_path / "tr.tsv").write_text("".join(lines_tr))
+    (tmp_path / "te.tsv").write_text("".join(lines_te))
+    out = tmp_path / "out"
+    out.mkdir()
+    nu, ni = lib.convert_tsv(tmp_path / "tr.tsv", tmp_path / "te.tsv", out)
+    uo, io = [], []
+    for ln in lines_tr + lines_te:
+        u, i = ln.split("\t")[:2]
+        if u not in uo:
+            uo.append(u)
+        if i not in io:
+            io.append(i)
+    r = random.Random(12345)
+    for v in (uo, io):
+        for i in reversed(range(1, len(v))):
+            j = int(r.random() * (i + 1))
+            v[i], v[j] = v[j], v[i]
+    assert (nu, ni) == (len(uo), len(io))
+    assert (out / "user_ids.txt").read_text() == "".join(f"{u}\n" for u in uo)
+    assert (out / "item_ids.txt").read_text() == "".join(f"{i}\n" for i in io)
+    ui = {u: q for q, u in enumerate(uo)}
+    ii = {i: q for q, i in enumerate(io)}
+    for name, lines in (("train.dat", lines_tr), ("test.dat", lines_te)):
+        t = [ln.split("\t") for ln in lines]
+        m = sp.csr_matrix(([float(x[2]) for x in t], ([ui[x[0]] for x in t], [ii[x[1]] for x in t])),
+                          shape=(len(uo), len(io)))
+        sizes = m.indptr[1:] - m.indptr[:-1]
+        want = (struct.pack("=iiii", 1211216, len(uo), len(io), m.getnnz()) + struct.pack(f"={len(sizes)}i", *sizes)
+                + struct.pack(f"={len(m.indices)}i", *m.indices) + struct.pack(f"={len(m.data)}d", *m.data))
+        assert (out / name).read_bytes() == want, name
+
+
+def test_tool_executables(tmp_path):
+    """The command-line front ends run under their reference names."""
+    b = ROOT / "nomad_amd" / "bin"
+    r = subprocess.run([str(b / "nomad_gen"), str(tmp_path / "g"), "60", "20", "4", "5"], capture_output=True,
+                       text=True)
+    assert r.returncode == 0 and "success" in r.stdout, r.stderr
+    a = lib.read_dat(tmp_path / "g" / "train.dat")
+    assert (a.m, a.n) == (60, 20) and a.nnz > 0
+    r = subprocess.run([str(b / "nomad_synthgen_conf"), "--nrow", "10", "--ncol", "5", "--rowfile",
+                        str(tmp_path / "r.txt"), "--colfile", str(tmp_path / "c.txt")], capture_output=True, text=True)
+    assert r.returncode == 0 and (tmp_path / "r.txt").exists(), r.stderr
+    r = subprocess.run([str(b / "nomad_convert")], capture_output=True, text=True)
+    assert r.returncode == 1 and "usage" in r.stderr
