@@ -264,6 +264,15 @@ int nomad_get_route_state(nomad_ctx* ctx, int64_t* vptr, int32_t* unit, int32_t*
  * in the entry order of nomad_get_route_state); nentries must equal V. */
 int nomad_set_visit_counts(nomad_ctx* ctx, const int32_t* count, int64_t nentries);
 
+/* Epoch progress of a context, for checkpoints: strat mode the completed
+ * B x B sweeps (the step-size count of the next one); async mode the route
+ * cycles every column has completed (what nomad_train(epochs) counts from).
+ * set also gives every column of an async context that many completed
+ * cycles.  The reference checkpoints no counter (nb_checkpoint.cpp:45-143);
+ * a resumed run needs it to continue the same step schedule. */
+int nomad_get_progress(nomad_ctx* ctx, int64_t* epochs);
+int nomad_set_progress(nomad_ctx* ctx, int64_t epochs);
+
 /* ---- CCD++ rank-one solver (the reference's competitor program) ----------
  * Replaces ccdr1_mpi at one process (Code/competitor/ccdr1-mpi-omp.cpp:398-558:
  * RankOneUpdate :334-358, UpdateRating :360-385, the rank / inner-iteration

@@ -110,6 +110,22 @@ int nomad_tools_synthgen_conf(int32_t nrow, int32_t ncol, int32_t seed, const ch
 int nomad_tools_synthgen(const char* row_path, const char* col_path, const char* out_dir, int32_t nrow,
                          int32_t ncol, int32_t dim, int32_t seed, int64_t* train_nnz, int64_t* test_nnz);
 
+/* Checkpoint files in the reference's layout (Code/nomad/nb_checkpoint.cpp:
+ * gen_cp_file_name :37-42, archive_local :45-56, _archive_msg_queue :58-85,
+ * ColumnData::serialize pool.cpp:17-23):
+ *   <dir>/cp-t<job_id>-e<epoch>-p<part>.state  int32 rows, int32 k, double W[rows*k]
+ *   <dir>/cp-t<job_id>-e<epoch>-p<part>.msg    packed records {int32 column,
+ *                                              int64 flag = 0, double h[k]}
+ *   <dir>/cp-t<job_id>-e<epoch>-p0.smsg        empty (no send queue at one rank)
+ * for parts 0..nparts-1 of the rows (ceil(m / nparts) rows each,
+ * nb_data.cpp:83-89); column j goes to part j mod nparts.  W (m x k) and H
+ * (n x k) are row-major double.  The reader (restore, nb_checkpoint.cpp:
+ * 116-143) takes whole records only and requires every column to appear. */
+int nomad_tools_write_checkpoint(const char* dir, int32_t job_id, int32_t epoch, int32_t nparts, int32_t m,
+                                 int32_t n, int32_t k, const double* W, const double* H);
+int nomad_tools_read_checkpoint(const char* dir, int32_t job_id, int32_t epoch, int32_t nparts, int32_t m, int32_t n,
+                                int32_t k, double* W, double* H);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,7 +45,7 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", f"-I{CSRC}"]
 
 HIP_SOURCES = ["sgd_kernels.hip", "async_kernel.hip", "eval_kernels.hip", "ccd.hip", "objective.hip", "capi.hip",
                "autotune.hip"]
-TOOLS_SOURCES = ["datfile.cpp", "synth.cpp", "convert.cpp", "tools_capi.cpp"]
+TOOLS_SOURCES = ["datfile.cpp", "synth.cpp", "convert.cpp", "checkpoint.cpp", "tools_capi.cpp"]
 # command-line front ends of libnomad_tools.so: one program, one name per reference tool
 TOOL_NAMES = ["nomad_convert", "nomad_synthgen_conf", "nomad_synthgen", "nomad_gen"]
 

@@ -1369,6 +1369,33 @@ int nomad_set_visit_counts(nomad_ctx* c, const int32_t* count, int64_t nentries)
     return NOMAD_OK;
 }
 
+int nomad_get_progress(nomad_ctx* c, int64_t* epochs) {
+    if (!c || !epochs) return fail(NOMAD_EINVAL, "nomad_get_progress: null argument");
+    *epochs = c->p.mode == NOMAD_MODE_STRAT ? c->strat_epochs : c->cycle_base;
+    return NOMAD_OK;
+}
+
+int nomad_set_progress(nomad_ctx* c, int64_t epochs) {
+    if (!c) return fail(NOMAD_EINVAL, "nomad_set_progress: null context");
+    if (epochs < 0 || epochs >= INT_MAX) return fail(NOMAD_EINVAL, "nomad_set_progress: epochs out of range");
+    if (c->p.mode == NOMAD_MODE_STRAT) {
+        c->strat_epochs = epochs;
+        return NOMAD_OK;
+    }
+    if (c->tr.nnz < 0) return fail(NOMAD_ESTATE, "nomad_set_progress: no training data uploaded");
+    NOMAD_HIP_TRY(hipSetDevice(c->p.device));
+    NOMAD_HIP_TRY(hipStreamSynchronize(c->stream));
+    int rc;
+    if (!c->routes_ready && (rc = build_async_state(c))) return rc;
+    const int32_t n = c->p.n;
+    std::vector<TokInfo> tok(static_cast<size_t>(n));
+    NOMAD_HIP_TRY(hipMemcpy(tok.data(), c->d_tok, size_t(n) * sizeof(TokInfo), hipMemcpyDeviceToHost));
+    for (auto& t : tok) t.cycle = int32_t(epochs);
+    NOMAD_HIP_TRY(hipMemcpy(c->d_tok, tok.data(), size_t(n) * sizeof(TokInfo), hipMemcpyHostToDevice));
+    c->cycle_base = epochs;
+    return NOMAD_OK;
+}
+
 int nomad_upload_train(nomad_ctx* c, const int64_t* colptr, const int32_t* rowidx, const double* val,
                        int64_t nnz) {
     if (!c) return fail(NOMAD_EINVAL, "nomad_upload_train: null context");

@@ -1,5 +1,5 @@
 // Rating-file tooling: TSV -> .dat conversion and the reference's synthetic
-// generators (see convert.cpp).
+// generators (convert.cpp), checkpoint files (checkpoint.cpp).
 #pragma once
 
 #include <cstdint>
@@ -34,5 +34,12 @@ bool synthgen_conf(int32_t nrow, int32_t ncol, int32_t seed, const std::string& 
 // 1015) from the degree files.
 bool synthgen(const std::string& row_path, const std::string& col_path, const std::string& out_dir, int32_t nrow,
               int32_t ncol, int32_t dim, int32_t seed, int64_t& train_nnz, int64_t& test_nnz, std::string& err);
+
+// Checkpoint files of the reference's layout (checkpoint.cpp): W in nparts
+// .state files, the columns as .msg records, an empty .smsg.
+bool write_checkpoint(const std::string& dir, int job, int epoch, int nparts, int32_t m, int32_t n, int32_t k,
+                      const double* W, const double* H, std::string& err);
+bool read_checkpoint(const std::string& dir, int job, int epoch, int nparts, int32_t m, int32_t n, int32_t k,
+                     double* W, double* H, std::string& err);
 
 }  // namespace nomad

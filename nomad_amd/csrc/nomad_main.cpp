@@ -9,10 +9,18 @@
 // (nb_main.cpp:683-705, 350-366).  Training runs through the C ABI of
 // libnomad_hip; the SGD threads of the reference are the GPU's waves.
 //
+// Checkpoints (--cptype other than none): after every timeout segment, with
+// every column at rest, the factors go to the reference's checkpoint files
+// (<cppath>/cp-t<jobid>-e<segment>-p<part>.state / .msg / .smsg,
+// nb_checkpoint.cpp:37-143, --nthreads parts) plus cp-t<jobid>-e<segment>.nomad
+// with the step-size state the reference does not save (the epoch count and,
+// in async mode, the per-(column, workgroup) visit counts); --recover <e>
+// starts from checkpoint e instead of the initial factors.
+//
 // Options the reference accepts for its multi-machine machinery (cpm, tpm,
-// termination check, checkpointing, network control) are accepted and have
-// no effect at one rank on one GPU.  GPU-specific options: --workers (row
-// blocks, 0: automatic), --device, --storage, --rounding, --mode, --balance.
+// termination check, network control) are accepted and have no effect at one
+// rank on one GPU.  GPU-specific options: --workers (row blocks, 0:
+// automatic), --device, --storage, --rounding, --mode, --balance.
 #include <cerrno>
 #include <chrono>
 #include <cmath>
@@ -52,6 +60,9 @@ struct Options {
     int rounding = NOMAD_ROUND_NEAREST;
     int mode = NOMAD_MODE_ASYNC;
     bool balance = true;
+    // checkpoints
+    std::string cp_type = "none", cp_path;
+    int job_id = 1, recover = -1;
 };
 
 struct Spec {
@@ -137,11 +148,12 @@ bool parse_args(int argc, char** argv, Options& o) {
          "for fp32/bf16 at 8 waves per workgroup, 64 at 4 or 16, 32 for fp64)",
          Spec::kInt, &o.ptoken},
         {"output", 'o', "path prefix of the factor dump (<output>0)", Spec::kString, &o.output},
-        {"recover", 0, "accepted, no effect", Spec::kIgnoredValue, &ignored_d},
-        {"jobid", 'j', "accepted, no effect", Spec::kIgnoredValue, &ignored_d},
-        {"cptype", 'c', "accepted, no effect", Spec::kIgnoredValue, &ignored_s},
-        {"cpinterval", 'k', "accepted, no effect", Spec::kIgnoredValue, &ignored_d},
-        {"cppath", 'p', "accepted, no effect", Spec::kIgnoredValue, &ignored_s},
+        {"recover", 0, "recover from the checkpoint of this segment (-1: do not recover)", Spec::kInt, &o.recover},
+        {"jobid", 'j', "job id in the checkpoint file names (1)", Spec::kInt, &o.job_id},
+        {"cptype", 'c', "checkpoint type: none, or sync / async / vs (a snapshot after every timeout segment)",
+         Spec::kString, &o.cp_type},
+        {"cpinterval", 'k', "accepted (checkpoints are taken at the timeouts)", Spec::kIgnoredValue, &ignored_d},
+        {"cppath", 'p', "directory of the checkpoint files", Spec::kString, &o.cp_path},
         {"net_delay", 0, "accepted, no effect", Spec::kIgnoredValue, &ignored_d},
         {"net_ratio", 0, "accepted, no effect", Spec::kIgnoredValue, &ignored_s},
         {"verbose", 'v', "verbose level", Spec::kInt, &o.verbose},
@@ -260,6 +272,14 @@ bool parse_args(int argc, char** argv, Options& o) {
                 break;
             }
         if (o.dim < 1 || o.dim > 128 || o.nthreads < 0 || o.ptoken < 0) bad = true;
+        if (o.cp_type != "none" && o.cp_type != "sync" && o.cp_type != "async" && o.cp_type != "vs") {
+            std::cerr << "checkpoint type must be none, sync, async or vs." << std::endl;
+            bad = true;
+        }
+        if (o.cp_type != "none" && o.cp_path.empty()) {
+            std::cerr << "checkpoint path has to be specified." << std::endl;
+            bad = true;
+        }
     }
     if (bad || help) {
         usage(specs);
@@ -278,6 +298,69 @@ struct Matrix {
 int fail(const char* what, const char* msg) {
     std::cerr << "error: " << what << ": " << msg << std::endl;
     return 1;
+}
+
+// cp-t<job>-e<epoch>.nomad: "NOMADCP1", int32 mode, storage, m, n, k, 0,
+// int64 epochs, int64 V, int32 visit counts[V] (V = 0 in strat mode)
+std::string state_name(const Options& o, int epoch) {
+    return o.cp_path + "/cp-t" + std::to_string(o.job_id) + "-e" + std::to_string(epoch) + ".nomad";
+}
+
+int save_checkpoint(const Options& o, nomad_ctx* ctx, int epoch, int32_t m, int32_t n, int k, std::vector<double>& W,
+                    std::vector<double>& H) {
+    if (nomad_get_factors(ctx, W.data(), H.data())) return fail("checkpoint", nomad_last_error());
+    const int parts = o.nthreads > 0 ? o.nthreads : 1;
+    if (nomad_tools_write_checkpoint(o.cp_path.c_str(), o.job_id, epoch, parts, m, n, k, W.data(), H.data()))
+        return fail("checkpoint", nomad_tools_last_error());
+    int64_t epochs = 0, V = 0;
+    std::vector<int32_t> counts;
+    if (nomad_get_progress(ctx, &epochs)) return fail("checkpoint", nomad_last_error());
+    if (o.mode == NOMAD_MODE_ASYNC) {
+        if (nomad_get_route_state(ctx, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, &V))
+            return fail("checkpoint", nomad_last_error());
+        counts.resize(size_t(V));
+        if (nomad_get_route_state(ctx, nullptr, nullptr, nullptr, counts.data(), nullptr, nullptr, nullptr, nullptr))
+            return fail("checkpoint", nomad_last_error());
+    }
+    std::ofstream f(state_name(o, epoch), std::ios::binary);
+    const int32_t head[6] = {o.mode, o.storage, m, n, k, 0};
+    f.write("NOMADCP1", 8);
+    f.write(reinterpret_cast<const char*>(head), sizeof(head));
+    f.write(reinterpret_cast<const char*>(&epochs), 8);
+    f.write(reinterpret_cast<const char*>(&V), 8);
+    f.write(reinterpret_cast<const char*>(counts.data()), std::streamsize(4 * counts.size()));
+    if (!f) return fail("checkpoint", "cannot write the .nomad state file");
+    std::cout << "checkpoint " << epoch << " written to " << o.cp_path << std::endl;
+    return 0;
+}
+
+// the step-size state of checkpoint `epoch`, if its .nomad file exists
+int restore_state(const Options& o, nomad_ctx* ctx, int32_t m, int32_t n, int k) {
+    std::ifstream f(state_name(o, o.recover), std::ios::binary);
+    if (!f) {
+        std::cout << "no " << state_name(o, o.recover) << ": step sizes restart from the first visit" << std::endl;
+        return 0;
+    }
+    char magic[8];
+    int32_t head[6];
+    int64_t epochs = 0, V = 0;
+    f.read(magic, 8);
+    f.read(reinterpret_cast<char*>(head), sizeof(head));
+    f.read(reinterpret_cast<char*>(&epochs), 8);
+    f.read(reinterpret_cast<char*>(&V), 8);
+    if (!f || std::memcmp(magic, "NOMADCP1", 8) != 0 || head[0] != o.mode || head[2] != m || head[3] != n ||
+        head[4] != k)
+        return fail("recover", "the .nomad state file does not match this run (mode or sizes)");
+    if (nomad_set_progress(ctx, epochs)) return fail("recover", nomad_last_error());
+    if (o.mode == NOMAD_MODE_ASYNC && V > 0) {
+        std::vector<int32_t> counts(static_cast<size_t>(V));
+        f.read(reinterpret_cast<char*>(counts.data()), std::streamsize(4 * V));
+        if (!f) return fail("recover", "truncated .nomad state file");
+        if (nomad_set_visit_counts(ctx, counts.data(), V))
+            return fail("recover", "the visit counts do not fit this run's workers (same --workers and --balance)");
+    }
+    std::cout << "recovered checkpoint " << o.recover << " at epoch " << epochs << std::endl;
+    return 0;
 }
 
 }  // namespace
@@ -332,6 +415,15 @@ int main(int argc, char** argv) {
                           nomad_tools_matrix_val(te.p), tnnz) ||
         nomad_set_factors(ctx, W.data(), H.data()))
         return done(fail("upload", nomad_last_error()));
+    if (o.recover >= 0) {
+        // the reference's restore (nb_checkpoint.cpp:116-143): W parts from
+        // .state, the columns from .msg, then this build's step-size state
+        const int parts = o.nthreads > 0 ? o.nthreads : 1;
+        if (nomad_tools_read_checkpoint(o.cp_path.c_str(), o.job_id, o.recover, parts, m, n, k, W.data(), H.data()))
+            return done(fail("recover", nomad_tools_last_error()));
+        if (nomad_set_factors(ctx, W.data(), H.data())) return done(fail("recover", nomad_last_error()));
+        if (restore_state(o, ctx, m, n, k)) return done(1);
+    }
     int32_t workers = 0;
     nomad_get_nworkers(ctx, &workers);
     std::cout << "workers: " << workers << std::endl;
@@ -380,6 +472,7 @@ int main(int argc, char** argv) {
             << ", c=" << es.test_count << "; u=" << updates << ", f=" << failures << ", s=" << 0 << "\n";
         buf << "=====================================================";
         std::cout << buf.str() << std::endl;
+        if (o.cp_type != "none" && save_checkpoint(o, ctx, int(it), m, n, k, W, H)) return done(1);
         if (o.pause) std::this_thread::sleep_for(std::chrono::duration<double>(3.0));
     }
 

@@ -306,4 +306,20 @@ int nomad_tools_synthgen(const char* row_path, const char* col_path, const char*
     return 0;
 }
 
+int nomad_tools_write_checkpoint(const char* dir, int32_t job_id, int32_t epoch, int32_t nparts, int32_t m,
+                                 int32_t n, int32_t k, const double* W, const double* H) {
+    if (!dir || !W || !H) return tfail("nomad_tools_write_checkpoint: null argument");
+    std::string err;
+    if (!nomad::write_checkpoint(dir, job_id, epoch, nparts, m, n, k, W, H, err)) return tfail(err);
+    return 0;
+}
+
+int nomad_tools_read_checkpoint(const char* dir, int32_t job_id, int32_t epoch, int32_t nparts, int32_t m, int32_t n,
+                                int32_t k, double* W, double* H) {
+    if (!dir || !W || !H) return tfail("nomad_tools_read_checkpoint: null argument");
+    std::string err;
+    if (!nomad::read_checkpoint(dir, job_id, epoch, nparts, m, n, k, W, H, err)) return tfail(err);
+    return 0;
+}
+
 }  // extern "C"

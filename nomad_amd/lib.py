@@ -115,6 +115,8 @@ HIP_SYMBOLS = {
     "nomad_get_counters": (C.c_int, [_P, _i64p, C.c_int32]),
     "nomad_get_route_state": (C.c_int, [_P, _i64p, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _i64p]),
     "nomad_set_visit_counts": (C.c_int, [_P, _i32p, C.c_int64]),
+    "nomad_get_progress": (C.c_int, [_P, _i64p]),
+    "nomad_set_progress": (C.c_int, [_P, C.c_int64]),
     "nomad_train_batch": (C.c_int, [C.POINTER(_P), C.c_int32, C.c_int64, C.c_int32, C.POINTER(TrainStats)]),
     "nomad_autotune": (C.c_int, [C.POINTER(Params), _i64p, _i32p, _f64p, C.c_int64, C.c_double, C.POINTER(TuneTrial),
                                  C.POINTER(TuneTrial), C.c_int32, _i32p]),
@@ -147,6 +149,10 @@ TOOLS_SYMBOLS = {
     "nomad_tools_synthgen_conf": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_char_p, C.c_char_p, C.c_char_p]),
     "nomad_tools_synthgen": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int32, C.c_int32, C.c_int32,
                                        C.c_int32, _i64p, _i64p]),
+    "nomad_tools_write_checkpoint": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                               C.c_int32, _f64p, _f64p]),
+    "nomad_tools_read_checkpoint": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                              C.c_int32, _f64p, _f64p]),
 }
 
 _libs: dict[str, C.CDLL] = {}
@@ -269,6 +275,23 @@ def synthgen(row_path, col_path, out_dir, nrow: int, ncol: int, dim: int, seed: 
     _tools_check(tools().nomad_tools_synthgen(str(row_path).encode(), str(col_path).encode(), str(out_dir).encode(),
                                               nrow, ncol, dim, seed, C.byref(a), C.byref(b)))
     return a.value, b.value
+
+
+def write_checkpoint(dir_path, job_id: int, epoch: int, nparts: int, W, H) -> None:
+    """W and H in the reference's checkpoint files (nomad_tools_write_checkpoint)."""
+    W = np.ascontiguousarray(W, np.float64)
+    H = np.ascontiguousarray(H, np.float64)
+    _tools_check(tools().nomad_tools_write_checkpoint(str(dir_path).encode(), job_id, epoch, nparts, W.shape[0],
+                                                      H.shape[0], W.shape[1], _ptr(W, C.c_double),
+                                                      _ptr(H, C.c_double)))
+
+
+def read_checkpoint(dir_path, job_id: int, epoch: int, nparts: int, m: int, n: int, k: int):
+    W = np.zeros((m, k))
+    H = np.zeros((n, k))
+    _tools_check(tools().nomad_tools_read_checkpoint(str(dir_path).encode(), job_id, epoch, nparts, m, n, k,
+                                                     _ptr(W, C.c_double), _ptr(H, C.c_double)))
+    return W, H
 
 
 def preset(config: int, scale: float = 1.0) -> SynthSpec:
@@ -481,6 +504,16 @@ class Context:
             _ptr(out["count"], C.c_int32), _ptr(out["route_pos"], C.c_int32), _ptr(out["tok_pos"], C.c_int32),
             _ptr(out["tok_cycle"], C.c_int32), None))
         return out
+
+    @property
+    def progress(self) -> int:
+        v = C.c_int64()
+        self._check(self._lib.nomad_get_progress(self._h, C.byref(v)))
+        return v.value
+
+    @progress.setter
+    def progress(self, epochs: int) -> None:
+        self._check(self._lib.nomad_set_progress(self._h, epochs))
 
     def set_visit_counts(self, count) -> None:
         c = np.ascontiguousarray(count, np.int32)

@@ -423,3 +423,52 @@ def test_tool_executables(tmp_path):
     assert r.returncode == 0 and (tmp_path / "r.txt").exists(), r.stderr
     r = subprocess.run([str(b / "nomad_convert")], capture_output=True, text=True)
     assert r.returncode == 1 and "usage" in r.stderr
+
+
+# ---- checkpoint files (Code/nomad/nb_checkpoint.cpp layout) --------------------
+def test_checkpoint_files_layout_and_round_trip(tmp_path):
+    """nomad_tools_write_checkpoint writes the reference's files: per part
+    p of ceil(m / P) rows a .state file (int32 rows, int32 dim, the rows as
+    doubles, archive_local) and a .msg file of packed column records (int32
+    column, int64 flag, dim doubles, ColumnData::serialize) with the columns
+    j = p (mod P); an empty .smsg for part 0.  Reading them back restores W
+    and H exactly."""
+    m, n, k, P = 10, 7, 4, 3
+    rng = np.random.default_rng(2)
+    W, H = rng.normal(size=(m, k)), rng.normal(size=(n, k))
+    lib.write_checkpoint(tmp_path, 5, 2, P, W, H)
+    per = -(-m // P)
+    for p in range(P):
+        base = tmp_path / f"cp-t5-e2-p{p}"
+        rows = W[p * per:min((p + 1) * per, m)]
+        assert (base.with_suffix(".state")).read_bytes() == struct.pack("<ii", rows.shape[0], k) + rows.tobytes()
+        want = b"".join(struct.pack("<iq", j, 0) + H[j].tobytes() for j in range(p, n, P))
+        assert (base.with_suffix(".msg")).read_bytes() == want
+    assert (tmp_path / "cp-t5-e2-p0.smsg").read_bytes() == b""
+    W2, H2 = lib.read_checkpoint(tmp_path, 5, 2, P, m, n, k)
+    assert np.array_equal(W2, W) and np.array_equal(H2, H)
+
+
+def test_checkpoint_reader_rejects_bad_files(tmp_path):
+    m, n, k = 6, 5, 3
+    W, H = np.ones((m, k)), np.ones((n, k))
+    lib.write_checkpoint(tmp_path, 1, 0, 2, W, H)
+    with pytest.raises(lib.NomadError, match="rows of dimension"):
+        lib.read_checkpoint(tmp_path, 1, 0, 2, m, n, k + 1)
+    msg = tmp_path / "cp-t1-e0-p1.msg"
+    raw = msg.read_bytes()
+    msg.write_bytes(raw[:-5])  # a truncated record
+    with pytest.raises(lib.NomadError, match="truncated"):
+        lib.read_checkpoint(tmp_path, 1, 0, 2, m, n, k)
+    rec = 12 + 8 * k
+    msg.write_bytes(raw[:len(raw) - rec])  # a column missing
+    with pytest.raises(lib.NomadError, match="no record of column"):
+        lib.read_checkpoint(tmp_path, 1, 0, 2, m, n, k)
+
+
+@pytest.mark.skipif(not DRIVER.exists(), reason="driver not built")
+def test_driver_checkpoint_options(tmp_path):
+    r = run_driver("-i", str(tmp_path), "--cptype", "sync")
+    assert r.returncode != 0 and "checkpoint path has to be specified" in r.stderr
+    r = run_driver("-i", str(tmp_path), "--cptype", "often", "--cppath", str(tmp_path))
+    assert r.returncode != 0 and "checkpoint type" in r.stderr
