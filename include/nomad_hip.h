@@ -298,7 +298,9 @@ typedef struct nomad_ccd_stats {
     double seconds;        /* wall time of the call                   */
     double alg_bytes;      /* algorithmic bytes: per inner iteration
                               2 nnz (4 + 8) + 16 (m + n); per rank
-                              32 (nnz + test nnz) (residual updates)  */
+                              32 (2 nnz + test nnz) (residual updates
+                              of the CSC and CSR copies and the test
+                              residual)                               */
     double solve_seconds;  /* wall time of the rank / inner-iteration
                               loop alone (after the host builds the
                               CSR, uploads and the residual init)     */

@@ -13,7 +13,16 @@
 // Column and row passes split every column (row) into tasks of at most
 // kTask ratings, one wave per task; the partial sums of a column are added in
 // task order, so every result is deterministic.  The row pass reads the
-// CSC-ordered residual through the CSR -> CSC position map.
+// residual's CSR-ordered copy, which every change of the residual updates as
+// well (the reference keeps R and Rt alike, :444-447, 511-512): read through
+// the CSR -> CSC position map instead, a row pass fetched 6.8x its bytes.
+//
+// The early stop is decided on the device: after each inner iteration one
+// thread compares fundec with fundec_max * eps and, when the rank is done,
+// sets a flag that makes the remaining queued iterations of the rank return
+// at once.  The host enqueues a rank's inner iterations without waiting and
+// reads back once per rank (the rank's objective and test RMSE, and the
+// count of first-iteration stops that ends the outer iteration).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,7 +41,15 @@ int fail(int code, const std::string& msg);
 namespace {
 
 constexpr int kTask = 2048;    // ratings per pass task
-constexpr int kRedBlocks = 256;  // blocks of the first reduction stage
+constexpr int kRedBlocks = 2048;  // blocks of the grid-stride passes and first reduction stage
+
+// early-stop state of the outer iteration (:421-423, 466-497)
+struct CcdState {
+    double fundec_max;
+    int32_t early_stop;  // ranks stopped at their first inner iteration
+    int32_t stopped;     // the current rank's inner iterations are over
+    int64_t inner_done;
+};
 
 struct CcdTask {
     int64_t e0;
@@ -78,19 +95,42 @@ __global__ void ccd_rank_update(int64_t nnz, const int32_t* rowidx, const int32_
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
+// the CSR-ordered copy of the residual: rt[d] = r[perm[d]] (once, at set-up)
+__global__ void ccd_gather(int64_t nnz, const int32_t* perm, const double* r, double* rt) {
+    for (int64_t d = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; d < nnz; d += int64_t(gridDim.x) * blockDim.x)
+        rt[d] = r[perm[d]];
+}
+
 // partial (g, h) of every task: one wave per task
-__global__ void ccd_pass_partial(const CcdTask* tasks, int ntasks, const int32_t* idx, const int32_t* perm,
-                                 const double* r, const double* x, double2* part) {
+__global__ void ccd_pass_partial(const CcdTask* tasks, int ntasks, const int32_t* idx, const double* r,
+                                 const double* x, double2* part, const CcdState* st) {
+    if (st->stopped) return;
     const int lane = threadIdx.x & 63;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int nwaves = (gridDim.x * blockDim.x) >> 6;
     for (int t = wave; t < ntasks; t += nwaves) {
         const CcdTask tk = tasks[t];
         double g = 0, h = 0;
-        for (int q = lane; q < tk.len; q += 64) {
+        // four of the lane's ratings in flight, added in the same order
+        int q = lane;
+        for (; q + 192 < tk.len; q += 256) {
+            double xv[4], rv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t e = tk.e0 + q + 64 * u;
+                xv[u] = x[idx[e]];
+                rv[u] = r[e];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                g += xv[u] * rv[u];
+                h += xv[u] * xv[u];
+            }
+        }
+        for (; q < tk.len; q += 64) {
             const int64_t e = tk.e0 + q;
             const double xi = x[idx[e]];
-            const double rv = perm ? r[perm[e]] : r[e];
+            const double rv = r[e];
             g += xi * rv;
             h += xi * xi;
         }
@@ -102,9 +142,9 @@ __global__ void ccd_pass_partial(const CcdTask* tasks, int ntasks, const int32_t
 
 // new value of every line from its tasks' partials, in task order
 __global__ void ccd_pass_finalize(int nlines, const int32_t* task_ptr, const int64_t* ptr, const double2* part,
-                                  double lambda, double* y, double* fundec) {
+                                  double lambda, double* y, double* fundec, const CcdState* st) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nlines) return;
+    if (c >= nlines || st->stopped) return;
     const int64_t cnt = ptr[c + 1] - ptr[c];
     if (cnt == 0) {  // RankOneUpdate returns 0 for an empty line
         y[c] = 0;
@@ -123,8 +163,10 @@ __global__ void ccd_pass_finalize(int nlines, const int32_t* task_ptr, const int
     y[c] = nv;
 }
 
-// sum of a[i] (or a[i]^2 (ptr[i+1] - ptr[i]) with ptr) : stage 1, one partial per block
-__global__ void ccd_sum_stage1(int64_t n, const double* a, const int64_t* ptr, double* part) {
+// sum of a[i] (or a[i]^2 (ptr[i+1] - ptr[i]) with ptr) : stage 1, one partial per
+// block (skipped while st->stopped, st may be null)
+__global__ void ccd_sum_stage1(int64_t n, const double* a, const int64_t* ptr, double* part, const CcdState* st) {
+    if (st && st->stopped) return;
     __shared__ double s[256];
     double acc = 0;
     for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
@@ -139,7 +181,8 @@ __global__ void ccd_sum_stage1(int64_t n, const double* a, const int64_t* ptr, d
 }
 
 // stage 2: one block adds the stage-1 partials; out[slot] = sum
-__global__ void ccd_sum_stage2(int n, const double* part, double* out, int slot) {
+__global__ void ccd_sum_stage2(int n, const double* part, double* out, int slot, const CcdState* st) {
+    if (st && st->stopped) return;
     __shared__ double s[256];
     double acc = 0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) acc += part[i];
@@ -150,6 +193,29 @@ __global__ void ccd_sum_stage2(int n, const double* part, double* out, int slot)
         __syncthreads();
     }
     if (threadIdx.x == 0) out[slot] = s[0];
+}
+
+// after inner iteration `iter` of rank t in outer iteration oiter: the
+// reference's test on fundec = out[0] + out[1] (ccdr1-mpi-omp.cpp:490-497)
+__global__ void ccd_inner_decide(CcdState* st, const double* out, int iter, int oiter, int t, double eps) {
+    if (st->stopped) return;
+    ++st->inner_done;
+    const double cur = out[0] + out[1];
+    if (cur < st->fundec_max * eps) {
+        if (iter == 1) st->early_stop += 1;
+        st->stopped = 1;
+        return;
+    }
+    if (!(oiter == 1 && t == 0 && iter == 1)) st->fundec_max = fmax(st->fundec_max, cur);
+}
+
+// rank start (stopped = 0) or outer-iteration start (everything reset)
+__global__ void ccd_state_reset(CcdState* st, int outer) {
+    if (outer) {
+        st->fundec_max = 0;
+        st->early_stop = 0;
+    }
+    st->stopped = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -193,7 +259,7 @@ struct Side {  // one of the two passes
     int ntasks = 0;
     const int64_t* ptr = nullptr;  // device
     const int32_t* idx = nullptr;
-    const int32_t* perm = nullptr;
+    const double* r = nullptr;  // residual in this side's order
     CcdTask* tasks = nullptr;
     int32_t* task_ptr = nullptr;
     double2* part = nullptr;
@@ -237,7 +303,9 @@ extern "C" int nomad_ccd(const nomad_ccd_params* p, const int64_t* colptr, const
     std::vector<int64_t> rowptr(size_t(m) + 1, 0);
     for (int64_t e = 0; e < nnz; ++e) rowptr[size_t(rowidx[e]) + 1]++;
     for (int i = 0; i < m; ++i) rowptr[size_t(i) + 1] += rowptr[size_t(i)];
-    std::vector<int32_t> colidx(static_cast<size_t>(nnz)), perm(static_cast<size_t>(nnz));
+    std::vector<int32_t> colidx(static_cast<size_t>(nnz)), perm(static_cast<size_t>(nnz)), erow(static_cast<size_t>(nnz));
+    for (int i = 0; i < m; ++i)
+        for (int64_t d = rowptr[size_t(i)]; d < rowptr[size_t(i) + 1]; ++d) erow[size_t(d)] = i;
     {
         std::vector<int64_t> fill(rowptr.begin(), rowptr.end() - 1);
         for (int j = 0; j < n; ++j)
@@ -261,15 +329,17 @@ extern "C" int nomad_ccd(const nomad_ccd_params* p, const int64_t* colptr, const
     Buffers b;
     int rc;
     int64_t *d_colptr, *d_rowptr;
-    int32_t *d_rowidx, *d_ecol, *d_colidx, *d_perm, *d_te_rowidx = nullptr, *d_te_ecol = nullptr;
-    double *d_val, *d_r, *d_te_val = nullptr, *d_te_r = nullptr, *d_Wt, *d_Ht, *d_part, *d_out;
+    int32_t *d_rowidx, *d_ecol, *d_colidx, *d_erow, *d_perm, *d_te_rowidx = nullptr, *d_te_ecol = nullptr;
+    double *d_val, *d_r, *d_rt, *d_te_val = nullptr, *d_te_r = nullptr, *d_Wt, *d_Ht, *d_part, *d_out;
+    CcdState* d_state = nullptr;
     Side cs, rs;
     if ((rc = b.upload(&d_colptr, colptr, size_t(n) + 1)) || (rc = b.upload(&d_rowidx, rowidx, size_t(nnz))) ||
         (rc = b.upload(&d_ecol, ecol.data(), ecol.size())) || (rc = b.upload(&d_val, val, size_t(nnz))) ||
         (rc = b.alloc(&d_r, size_t(nnz))) || (rc = b.upload(&d_rowptr, rowptr.data(), rowptr.size())) ||
         (rc = b.upload(&d_colidx, colidx.data(), colidx.size())) || (rc = b.upload(&d_perm, perm.data(), perm.size())) ||
+        (rc = b.upload(&d_erow, erow.data(), erow.size())) || (rc = b.alloc(&d_rt, size_t(nnz))) ||
         (rc = b.upload(&d_Wt, Wt.data(), Wt.size())) || (rc = b.upload(&d_Ht, Ht.data(), Ht.size())) ||
-        (rc = b.alloc(&d_part, kRedBlocks)) || (rc = b.alloc(&d_out, 8)) ||
+        (rc = b.alloc(&d_part, kRedBlocks)) || (rc = b.alloc(&d_out, 8)) || (rc = b.alloc(&d_state, 1)) ||
         (rc = b.upload(&cs.tasks, ctasks.data(), ctasks.size())) ||
         (rc = b.upload(&cs.task_ptr, ctask_ptr.data(), ctask_ptr.size())) ||
         (rc = b.alloc(&cs.part, ctasks.size())) || (rc = b.alloc(&cs.fundec, size_t(n))) ||
@@ -285,11 +355,12 @@ extern "C" int nomad_ccd(const nomad_ccd_params* p, const int64_t* colptr, const
     cs.ntasks = int(ctasks.size());
     cs.ptr = d_colptr;
     cs.idx = d_rowidx;
+    cs.r = d_r;
     rs.nlines = m;
     rs.ntasks = int(rtasks.size());
     rs.ptr = d_rowptr;
     rs.idx = d_colidx;
-    rs.perm = d_perm;
+    rs.r = d_rt;
 
     hipStream_t s = nullptr;
     if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
@@ -307,15 +378,20 @@ extern "C" int nomad_ccd(const nomad_ccd_params* p, const int64_t* colptr, const
     if (has_test)
         hipLaunchKernelGGL(ccd_residual_init, dim3(grid_for(te_nnz, eb)), dim3(eb), 0, s, te_nnz, m, n, k,
                            d_te_rowidx, d_te_ecol, d_te_val, d_Wt, d_Ht, d_te_r);
+    hipLaunchKernelGGL(ccd_gather, dim3(grid_for(nnz, eb)), dim3(eb), 0, s, int64_t(nnz), d_perm, d_r, d_rt);
+    hipLaunchKernelGGL(ccd_state_reset, dim3(1), dim3(1), 0, s, d_state, 1);
 
     // out[slot] = sum; then read back all slots that were written
-    auto sum_into = [&](int64_t count, const double* a, const int64_t* ptr, int slot) {
-        hipLaunchKernelGGL(ccd_sum_stage1, dim3(kRedBlocks), dim3(256), 0, s, count, a, ptr, d_part);
-        hipLaunchKernelGGL(ccd_sum_stage2, dim3(1), dim3(256), 0, s, kRedBlocks, d_part, d_out, slot);
+    auto sum_into = [&](int64_t count, const double* a, const int64_t* ptr, int slot,
+                        const CcdState* skip = nullptr) {
+        hipLaunchKernelGGL(ccd_sum_stage1, dim3(kRedBlocks), dim3(256), 0, s, count, a, ptr, d_part, skip);
+        hipLaunchKernelGGL(ccd_sum_stage2, dim3(1), dim3(256), 0, s, kRedBlocks, d_part, d_out, slot, skip);
     };
     double host_out[8];
+    CcdState host_st{};
     auto fetch = [&]() -> int {
         if (hipMemcpyAsync(host_out, d_out, sizeof(host_out), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipMemcpyAsync(&host_st, d_state, sizeof(host_st), hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess)
             return fail(NOMAD_EHIP, "nomad_ccd: kernel or copy failed");
         return NOMAD_OK;
@@ -324,13 +400,15 @@ extern "C" int nomad_ccd(const nomad_ccd_params* p, const int64_t* colptr, const
                            double sign, double* r, bool want_loss, int slot) {
         hipLaunchKernelGGL(ccd_rank_update, dim3(kRedBlocks), dim3(256), 0, s, count, ri, ci, u, v, sign, r,
                            want_loss ? d_part : nullptr);
-        if (want_loss) hipLaunchKernelGGL(ccd_sum_stage2, dim3(1), dim3(256), 0, s, kRedBlocks, d_part, d_out, slot);
+        if (want_loss)
+            hipLaunchKernelGGL(ccd_sum_stage2, dim3(1), dim3(256), 0, s, kRedBlocks, d_part, d_out, slot,
+                               static_cast<const CcdState*>(nullptr));
     };
     auto pass = [&](const Side& sd, const double* x, double* y) {
         hipLaunchKernelGGL(ccd_pass_partial, dim3(grid_for(int64_t(sd.ntasks) * 64, 256)), dim3(256), 0, s,
-                           sd.tasks, sd.ntasks, sd.idx, sd.perm, d_r, x, sd.part);
+                           sd.tasks, sd.ntasks, sd.idx, sd.r, x, sd.part, d_state);
         hipLaunchKernelGGL(ccd_pass_finalize, dim3((sd.nlines + 255) / 256), dim3(256), 0, s, sd.nlines, sd.task_ptr,
-                           sd.ptr, sd.part, p->lambda, y, sd.fundec);
+                           sd.ptr, sd.part, p->lambda, y, sd.fundec, d_state);
     };
 
     // reg = sum_t sum_i W_it^2 nnz_i + sum_j H_jt^2 nnz_j
@@ -349,37 +427,35 @@ extern "C" int nomad_ccd(const nomad_ccd_params* p, const int64_t* colptr, const
     if ((rc = fetch())) return rc;  // set-up work finished
     const auto t_solve = std::chrono::steady_clock::now();
     for (int oiter = 1; oiter <= p->iters; ++oiter) {
-        double fundec_max = 0;  // both restart with every outer iteration
-        int early_stop = 0;
+        // fundec_max and the early-stop count restart with every outer iteration
+        hipLaunchKernelGGL(ccd_state_reset, dim3(1), dim3(1), 0, s, d_state, 1);
+        host_st.early_stop = 0;
         for (int t = 0; t < k; ++t) {
-            if (early_stop >= 5) break;
+            if (host_st.early_stop >= 5) break;
             double* u = d_Wt + size_t(t) * m;
             double* v = d_Ht + size_t(t) * n;
+            hipLaunchKernelGGL(ccd_state_reset, dim3(1), dim3(1), 0, s, d_state, 0);
             rank_update(nnz, d_rowidx, d_ecol, u, v, 1.0, d_r, false, 0);
+            rank_update(nnz, d_erow, d_colidx, u, v, 1.0, d_rt, false, 0);
             if (has_test) rank_update(te_nnz, d_te_rowidx, d_te_ecol, u, v, 1.0, d_te_r, false, 0);
-            sum_into(m, u, d_rowptr, 0);
-            sum_into(n, v, d_colptr, 1);
-            if ((rc = fetch())) return rc;
-            reg -= host_out[0] + host_out[1];
+            sum_into(m, u, d_rowptr, 4);
+            sum_into(n, v, d_colptr, 5);
+            // all inner iterations queued at once; the device stops them
             for (int iter = 1; iter <= p->inner; ++iter) {
                 pass(cs, u, v);
                 pass(rs, v, u);
-                sum_into(n, cs.fundec, nullptr, 0);
-                sum_into(m, rs.fundec, nullptr, 1);
-                if ((rc = fetch())) return rc;
-                ++inner_done;
-                const double fundec_cur = host_out[0] + host_out[1];
-                if (fundec_cur < fundec_max * p->eps) {
-                    if (iter == 1) early_stop += 1;
-                    break;
-                }
-                if (!(oiter == 1 && t == 0 && iter == 1)) fundec_max = std::max(fundec_max, fundec_cur);
+                sum_into(n, cs.fundec, nullptr, 0, d_state);
+                sum_into(m, rs.fundec, nullptr, 1, d_state);
+                hipLaunchKernelGGL(ccd_inner_decide, dim3(1), dim3(1), 0, s, d_state, d_out, iter, oiter, t, p->eps);
             }
             rank_update(nnz, d_rowidx, d_ecol, u, v, -1.0, d_r, true, 2);
+            rank_update(nnz, d_erow, d_colidx, u, v, -1.0, d_rt, false, 0);
             if (has_test) rank_update(te_nnz, d_te_rowidx, d_te_ecol, u, v, -1.0, d_te_r, true, 3);
             sum_into(m, u, d_rowptr, 0);
             sum_into(n, v, d_colptr, 1);
-            if ((rc = fetch())) return rc;
+            if ((rc = fetch())) return rc;  // the one read-back of the rank
+            inner_done = host_st.inner_done;
+            reg -= host_out[4] + host_out[5];
             reg += host_out[0] + host_out[1];
             const int64_t q = int64_t(oiter - 1) * k + t;
             if (obj) obj[q] = host_out[2] + reg * p->lambda;
@@ -404,7 +480,7 @@ extern "C" int nomad_ccd(const nomad_ccd_params* p, const int64_t* colptr, const
         // rating and write both vectors; per rank: residual read + write for
         // the add and the subtract, training and test
         st->alg_bytes = double(inner_done) * (2.0 * double(nnz) * 12.0 + double(m + n) * 16.0) +
-                        double(ranks_done) * 2.0 * double(nnz + (has_test ? te_nnz : 0)) * 16.0;
+                        double(ranks_done) * 2.0 * double(2 * nnz + (has_test ? te_nnz : 0)) * 16.0;
     }
     return NOMAD_OK;
 }
