@@ -170,6 +170,76 @@ __device__ __forceinline__ void run_visit_pre(const SgdArgs& a, const RowBlock& 
     }
 }
 
+// ---- a row block of W held in LDS for a whole tile -----------------------------
+// Row r of the block is at wl[(r - rlo) * npairs + lane] as the stored pair
+// (the packed bf16 word, or the fp32 / fp64 pair): what the row's HBM words
+// would hold, so a rating reads exactly the value the previous update of the
+// row stored.  Lanes past the row read zero and store nothing, as the
+// buffer accesses' range check does.
+constexpr int kTileLdsBytes = 8192;  // per wave
+
+template <int S>
+__device__ __forceinline__ typename Store<S>::raw lds_row_load(const typename Store<S>::raw* wl, int32_t off,
+                                                               bool inr) {
+    typename Store<S>::raw z{};
+    return inr ? wl[off] : z;
+}
+
+// run_chunk with the rows in LDS: the same steps in the same order (a
+// column's rows are distinct, so a row loaded ahead is never stale), the
+// stored forms written back to LDS.
+template <int S, int RM>
+__device__ __forceinline__ void run_chunk_lds(const SgdArgs& a, typename Store<S>::raw* wl, int32_t rlo, int npairs,
+                                              typename Store<S>::pair& h, int32_t myrow,
+                                              typename Store<S>::rval myval, int32_t cnt,
+                                              typename Store<S>::acc step, typename Store<S>::acc reg,
+                                              uint32_t sr_b, uint32_t lkey) {
+    using T = Store<S>;
+    using Aa = typename T::acc;
+    using P = typename T::pair;
+    constexpr int kD = 4;
+    const int lane = lane_id();
+    const bool inr = lane < npairs;
+    typename T::raw wp[kD];
+#pragma unroll
+    for (int d = 0; d < kD; ++d)
+        wp[d] = lds_row_load<S>(wl, (readlane_i32(myrow, min(d, cnt - 1)) - rlo) * npairs + lane, inr);
+    for (int32_t t0 = 0;; t0 += kD) {
+#pragma unroll
+        for (int d = 0; d < kD; ++d) {
+            const int32_t t = t0 + d;
+            if (t >= cnt) goto chunk_done;
+            const int32_t i = readlane_i32(myrow, t);
+            P w = T::unpack(wp[d]);
+            sgd_step<P, Aa>(w, h, Aa(readlane_val(myval, t)), step, reg);
+            const uint32_t rnd = sr_bits_k(a.sr_seed, uint32_t(i), sr_b, lkey);
+            if (inr) wl[(i - rlo) * npairs + lane] = T::pack(w, rnd, RM);
+            wp[d] = lds_row_load<S>(wl, (readlane_i32(myrow, min(t + kD, cnt - 1)) - rlo) * npairs + lane, inr);
+        }
+    }
+chunk_done:;
+}
+
+template <int S, int RM>
+__device__ __forceinline__ void run_visit_lds(const SgdArgs& a, typename Store<S>::raw* wl, int32_t rlo, int npairs,
+                                              typename Store<S>::pair& h, int32_t j, int64_t start, int32_t len,
+                                              int32_t row0, typename Store<S>::rval val0,
+                                              typename Store<S>::acc step, typename Store<S>::acc reg,
+                                              uint32_t count, uint32_t lkey) {
+    using R = typename Store<S>::rval;
+    const int lane = lane_id();
+    const int32_t* __restrict__ ridx = a.rowidx + start;
+    const R* __restrict__ rv = static_cast<const R*>(a.val) + start;
+    const uint32_t sr_b = uint32_t(j) * 0x01000193u + count;
+    run_chunk_lds<S, RM>(a, wl, rlo, npairs, h, row0, val0, min(kWave, len), step, reg, sr_b, lkey);
+    for (int32_t base = kWave; base < len; base += kWave) {
+        const int32_t cnt = min(kWave, len - base);
+        const int32_t myrow = ridx[base + min(lane, cnt - 1)];
+        const R myval = rv[base + min(lane, cnt - 1)];
+        run_chunk_lds<S, RM>(a, wl, rlo, npairs, h, myrow, myval, cnt, step, reg, sr_b, lkey);
+    }
+}
+
 __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
     const uint32_t lo = __builtin_amdgcn_readfirstlane(int(uint32_t(v)));
     const uint32_t hi = __builtin_amdgcn_readfirstlane(int(uint32_t(v >> 32)));

@@ -34,16 +34,22 @@ using nomad_k::VisitDesc;
 // epoch, as with the reference's per-thread col_update_counts at one rank).
 // The tile's non-empty column visits are listed ahead (tile_ptr / visits), so
 // empty (column, row block) pairs cost nothing.  While a visit runs, the next
-// one's descriptor, h row and first 64 rows / values are already being
-// fetched: its column differs from the running one's (a column is visited
-// once per tile), so h is not being written; its W rows may be, and are
-// loaded only when the visit starts.  After the tile's last visit the
-// look-ahead reads the next tile's first visit or the list's sentinel
-// (length 0, start 0), in bounds, and drops it.
+// two visits' h rows and first 64 rows / values are already being fetched:
+// their columns differ from the running one's (a column is visited once per
+// tile), so their h is not being written; their W rows may be, and are
+// loaded only when the visit starts.
 // The single-problem kernel and the batched kernel both run a tile through
 // this function, so a problem's arithmetic is the same in either launch.
+//
+// When the tile's row block of W fits in the wave's LDS buffer (lds, with
+// room for lds_pairs stored pairs), the block is copied there at the start,
+// the tile's ratings read and write their rows there, and the block is copied
+// back at the end: the rows' values, and the order in which the steps see
+// them, are those of the HBM path, so the result is the same bit for bit,
+// but a step waits for LDS instead of HBM.
 template <int S, bool DC, int RM>
-__device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep, int epoch) {
+__device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep, int epoch,
+                                           typename Store<S>::raw* lds, int lds_pairs) {
     using T = Store<S>;
     using E = typename T::elem;
     using Aa = typename T::acc;
@@ -59,28 +65,94 @@ __device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep,
     E* __restrict__ H = static_cast<E*>(a.H);
     const R* __restrict__ rv = static_cast<const R*>(a.val);
     const int npairs = a.ldf >> 1;
-    const RowBlock rb = row_block<S>(a, b * a.rpb, min(a.m, (b + 1) * a.rpb), lane);
+    const int32_t rlo = b * a.rpb, rhi = min(a.m, (b + 1) * a.rpb);
+    const RowBlock rb = row_block<S>(a, rlo, rhi, lane);
     const uint32_t lkey = sr_lane_key(a.sr_seed, uint32_t(lane));
-    VisitDesc d = a.visits[v0];
-    typename T::pair h = row_load<S, DC>(H + int64_t(d.worker) * a.ldf, lane, npairs);
-    int64_t fi = d.start + min(lane, max(d.len, 1) - 1);
-    int32_t row = a.rowidx[fi];
-    R val = rv[fi];
+    const int32_t block_pairs = (rhi - rlo) * npairs;
+    const bool in_lds = lds != nullptr && block_pairs <= lds_pairs;
+    constexpr int CP = DC ? kCpDevice : 0;
+    if (in_lds) {
+        // the block's rows are contiguous in W: block_pairs stored pairs,
+        // eight per lane in flight
+        constexpr int kU = 8;
+        for (int32_t q0 = 0; q0 < block_pairs; q0 += kU * kWave) {
+            typename T::raw t[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) t[u] = T::template load_raw_b<CP>(rb.rsrc, (q0 + u * kWave + lane) * T::kPairBytes);
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+                if (q0 + u * kWave + lane < block_pairs) lds[q0 + u * kWave + lane] = t[u];
+        }
+    }
+    // Visit descriptors 64 at a time, one per lane (the current batch and
+    // the next), so that no descriptor costs a memory round trip inside the
+    // loop; a visit's column factor and its first 64 rows and values are
+    // requested two visits ahead.  Indexes past the tile are clamped to v1
+    // (the next tile's first visit or the list's sentinel, in bounds) and
+    // what they load is dropped.
+    struct Descs {
+        int64_t start;
+        int32_t len, col;
+    };
+    auto load_descs = [&](int64_t vb) {
+        const VisitDesc dd = a.visits[min(vb + lane, v1)];
+        return Descs{dd.start, dd.len, dd.worker};
+    };
+    int64_t vb = v0;
+    Descs dA = load_descs(vb), dB = load_descs(vb + kWave);
+    auto desc = [&](int64_t v, int64_t& st, int32_t& ln, int32_t& col) {
+        const int u = int(min(v, v1) - vb);
+        const Descs& q = u < kWave ? dA : dB;
+        const int l = u & (kWave - 1);
+        st = int64_t((uint64_t(uint32_t(readlane_i32(int32_t(uint64_t(q.start) >> 32), l))) << 32) |
+                     uint32_t(readlane_i32(int32_t(q.start), l)));
+        ln = readlane_i32(q.len, l);
+        col = readlane_i32(q.col, l);
+    };
+    struct Pre {
+        typename T::pair h;
+        int32_t row;
+        R val;
+    };
+    auto prefetch = [&](int64_t v) {
+        int64_t st;
+        int32_t ln, col;
+        desc(v, st, ln, col);
+        Pre p;
+        p.h = row_load<S, DC>(H + int64_t(col) * a.ldf, lane, npairs);
+        const int64_t fi = st + min(lane, max(ln, 1) - 1);
+        p.row = a.rowidx[fi];
+        p.val = rv[fi];
+        return p;
+    };
+    Pre c0 = prefetch(v0), c1 = prefetch(v0 + 1);
     for (int64_t v = v0; v < v1; ++v) {
-        const VisitDesc nd = a.visits[v + 1];
-        const typename T::pair hn = row_load<S, DC>(H + int64_t(nd.worker) * a.ldf, lane, npairs);
-        const int64_t ni = nd.start + min(lane, max(nd.len, 1) - 1);
-        const int32_t rown = a.rowidx[ni];
-        const R valn = rv[ni];
-        const int32_t j = d.worker;
-        run_visit_pre<S, DC, RM>(a, rb, h, j, d.start, d.len, row, val, step, reg, uint32_t(epoch), lkey);
+        if (v - vb == kWave) {
+            vb += kWave;
+            dA = dB;
+            dB = load_descs(vb + kWave);
+        }
+        const Pre c2 = prefetch(v + 2);
+        int64_t st;
+        int32_t ln, j;
+        desc(v, st, ln, j);
+        typename T::pair h = c0.h;
+        if (in_lds)
+            run_visit_lds<S, RM>(a, lds, rlo, npairs, h, j, st, ln, c0.row, c0.val, step, reg, uint32_t(epoch), lkey);
+        else
+            run_visit_pre<S, DC, RM>(a, rb, h, j, st, ln, c0.row, c0.val, step, reg, uint32_t(epoch), lkey);
         const uint32_t rnd = sr_bits_k(a.sr_seed, 0x80000000u | uint32_t(b),
                                        uint32_t(j) * 0x01000193u + uint32_t(epoch), lkey);
         row_store<S, DC>(H + int64_t(j) * a.ldf, lane, npairs, h, rnd, RM);
-        d = nd;
-        h = hn;
-        row = rown;
-        val = valn;
+        c0 = c1;
+        c1 = c2;
+    }
+    if (in_lds) {
+        // the block back to W (the range check drops the lanes past it)
+        for (int32_t q = lane; q < block_pairs + kWave - 1; q += kWave) {
+            const int32_t qq = min(q, block_pairs - 1);
+            T::template store_raw_b<CP>(rb.rsrc, q < block_pairs ? qq * T::kPairBytes : rb.drop(), lds[qq]);
+        }
     }
 }
 
@@ -88,9 +160,12 @@ __device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep,
 // branches on it; fp32 and fp64 ignore it)
 template <int S, int RM>
 __global__ __launch_bounds__(64) void sgd_strat_kernel(SgdArgs a) {
+    using raw = typename Store<S>::raw;
+    constexpr int kPairs = kTileLdsBytes / int(sizeof(raw));
+    __shared__ raw wl[kPairs];
     const int b = blockIdx.x;
     if (b >= a.nblocks) return;
-    strat_tile<S, false, RM>(a, b, a.substep, a.epoch);
+    strat_tile<S, false, RM>(a, b, a.substep, a.epoch, wl, kPairs);
 }
 
 // ---------------------------------------------------------------------------
@@ -125,6 +200,9 @@ __global__ __launch_bounds__(256) void sgd_batch_kernel(BatchArgs ba) {
     __shared__ int32_t seg_w[kBatchMax];
     __shared__ int32_t boff[kBatchMax + 1];
     __shared__ int32_t order[kBatchMax];
+    using raw = typename Store<S>::raw;
+    constexpr int kPairs = kTileLdsBytes / int(sizeof(raw));
+    __shared__ raw wl[4][kPairs];  // per wave (256-thread workgroups)
     const int Q = ba.nsorted, G = ba.nseg;
     for (int i = threadIdx.x; i <= Q; i += blockDim.x) {
         if (i < G) {
@@ -137,6 +215,7 @@ __global__ __launch_bounds__(256) void sgd_batch_kernel(BatchArgs ba) {
     }
     __syncthreads();
     const int lane = lane_id();
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
     const uint64_t t_end = wall_clock64() + ba.deadline;
     for (;;) {
         uint64_t t = 0;
@@ -177,11 +256,11 @@ __global__ __launch_bounds__(256) void sgd_batch_kernel(BatchArgs ba) {
         }
         if constexpr (S == NOMAD_STORE_BF16) {  // problems may round differently
             if (a.round_mode == NOMAD_ROUND_STOCHASTIC)
-                strat_tile<S, true, NOMAD_ROUND_STOCHASTIC>(a, b, r % B, a.epoch + r / B);
+                strat_tile<S, true, NOMAD_ROUND_STOCHASTIC>(a, b, r % B, a.epoch + r / B, wl[wave], kPairs);
             else
-                strat_tile<S, true, NOMAD_ROUND_NEAREST>(a, b, r % B, a.epoch + r / B);
+                strat_tile<S, true, NOMAD_ROUND_NEAREST>(a, b, r % B, a.epoch + r / B, wl[wave], kPairs);
         } else {
-            strat_tile<S, true, NOMAD_ROUND_NEAREST>(a, b, r % B, a.epoch + r / B);
+            strat_tile<S, true, NOMAD_ROUND_NEAREST>(a, b, r % B, a.epoch + r / B, wl[wave], kPairs);
         }
         wait_all_mem();
         if (lane == 0) st_dc(done + b, uint32_t(r + 1));
