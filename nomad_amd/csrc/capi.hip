@@ -39,6 +39,14 @@ template <int S>
 __global__ void sse_kernel(SseArgs);
 __global__ void seed_rings_kernel(SgdArgs, const int32_t*, const int64_t*);
 
+// dst[i] = src[idx[i]] (a permutation of the ratings' values)
+template <class R>
+__global__ void gather_kernel(const R* __restrict__ src, const int64_t* __restrict__ idx, R* __restrict__ dst,
+                              int64_t n) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        dst[i] = src[idx[i]];
+}
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -337,6 +345,11 @@ struct nomad_ctx {
     bool tiles_ready = false;
     int64_t* d_tile_ptr = nullptr;
     VisitDesc* d_tile_visits = nullptr;
+    int64_t* d_trow_ptr = nullptr;  // per tile its distinct rows ...
+    int32_t* d_trows = nullptr;
+    uint16_t* d_tslot = nullptr;    // ... and each rating's rank among them
+    int32_t* d_tm_row = nullptr;    // the training ratings in tile order (strat)
+    char* d_tm_val = nullptr;
 
     ~nomad_ctx() {
         dev_free(dW);
@@ -356,6 +369,11 @@ struct nomad_ctx {
         dev_free(d_vcnt);
         dev_free(d_tile_ptr);
         dev_free(d_tile_visits);
+        dev_free(d_trow_ptr);
+        dev_free(d_trows);
+        dev_free(d_tslot);
+        dev_free(d_tm_row);
+        dev_free(d_tm_val);
         dev_free(d_wbound);
         coh_free(d_tok);
         coh_free(d_tok_stamp);
@@ -482,12 +500,100 @@ int ensure_tiles(nomad_ctx* c) {
     }
     tptr[size_t(B) * B] = int64_t(vis.size());
     vis.push_back(VisitDesc{0, 0, 0});
+    // Each tile's distinct rows (ascending) and every rating's rank among
+    // them: a kernel may hold just those rows in LDS for the tile.  Ranks
+    // past 65535 are stored as 65535 (such a tile never fits in LDS).
+    const int64_t nnz = c->tr.nnz;
+    std::vector<int64_t> trp(size_t(B) * B + 1, 0);
+    std::vector<int32_t> trows;
+    std::vector<uint16_t> tslot(static_cast<size_t>(std::max<int64_t>(nnz, 1)), 0);
+    {
+        std::vector<int32_t> mark(static_cast<size_t>(c->rpb), -1), rows_t;
+        for (int32_t t = 0; t < B * B; ++t) {
+            const int32_t rlo = (t / B) * c->rpb;
+            trp[size_t(t)] = int64_t(trows.size());
+            rows_t.clear();
+            for (int64_t v = tptr[size_t(t)]; v < tptr[size_t(t) + 1]; ++v)
+                for (int64_t e = vis[size_t(v)].start; e < vis[size_t(v)].start + vis[size_t(v)].len; ++e) {
+                    int32_t& mk = mark[size_t(c->h_rowidx[size_t(e)] - rlo)];
+                    if (mk < 0) {
+                        mk = 0;
+                        rows_t.push_back(c->h_rowidx[size_t(e)]);
+                    }
+                }
+            std::sort(rows_t.begin(), rows_t.end());
+            for (size_t q = 0; q < rows_t.size(); ++q) mark[size_t(rows_t[q] - rlo)] = int32_t(q);
+            for (int64_t v = tptr[size_t(t)]; v < tptr[size_t(t) + 1]; ++v)
+                for (int64_t e = vis[size_t(v)].start; e < vis[size_t(v)].start + vis[size_t(v)].len; ++e)
+                    tslot[size_t(e)] = uint16_t(std::min(mark[size_t(c->h_rowidx[size_t(e)] - rlo)], 65535));
+            for (int32_t r : rows_t) mark[size_t(r - rlo)] = -1;
+            trows.insert(trows.end(), rows_t.begin(), rows_t.end());
+        }
+        trp[size_t(B) * B] = int64_t(trows.size());
+    }
+    // The ratings in tile order (tile after tile, visit after visit, rows
+    // ascending), so that consecutive visits' ratings are contiguous; the
+    // visits then index this order, and the sentinel's start is its end.
+    // One padding entry lets a kernel read one past the last rating.
+    const size_t ntm = static_cast<size_t>(nnz) + 1;
+    std::vector<int64_t> perm(ntm, 0);  // tile-order position -> CSC position
+    std::vector<int32_t> tm_row(ntm, 0);
+    std::vector<uint16_t> tm_slot(ntm, 0);
+    {
+        int64_t o = 0;
+        for (size_t v = 0; v + 1 < vis.size(); ++v) {
+            const int64_t s0 = vis[v].start;
+            vis[v].start = o;
+            for (int64_t e = s0; e < s0 + vis[v].len; ++e, ++o) {
+                perm[size_t(o)] = e;
+                tm_row[size_t(o)] = c->h_rowidx[size_t(e)];
+                tm_slot[size_t(o)] = tslot[size_t(e)];
+            }
+        }
+        vis.back().start = o;
+    }
     dev_free(c->d_tile_ptr);
     dev_free(c->d_tile_visits);
+    dev_free(c->d_trow_ptr);
+    dev_free(c->d_trows);
+    dev_free(c->d_tslot);
+    dev_free(c->d_tm_row);
+    dev_free(c->d_tm_val);
     int rc;
-    if ((rc = dev_alloc(&c->d_tile_ptr, tptr.size())) || (rc = dev_alloc(&c->d_tile_visits, vis.size()))) return rc;
+    if ((rc = dev_alloc(&c->d_tile_ptr, tptr.size())) || (rc = dev_alloc(&c->d_tile_visits, vis.size())) ||
+        (rc = dev_alloc(&c->d_trow_ptr, trp.size())) || (rc = dev_alloc(&c->d_trows, std::max<size_t>(trows.size(), 1))) ||
+        (rc = dev_alloc(&c->d_tslot, ntm)) || (rc = dev_alloc(&c->d_tm_row, ntm)) ||
+        (rc = dev_alloc(&c->d_tm_val, ntm * size_t(c->rvsz))))
+        return rc;
+    {
+        int64_t* d_perm = nullptr;
+        if ((rc = dev_alloc(&d_perm, ntm))) return rc;
+        NOMAD_HIP_TRY(hipMemcpy(d_perm, perm.data(), ntm * 8, hipMemcpyHostToDevice));
+        const int threads = 256;
+        const int blocks = int(std::min<int64_t>((int64_t(ntm) + threads - 1) / threads, 8192));
+        if (nnz > 0) {
+            if (c->rvsz == 8)
+                hipLaunchKernelGGL(gather_kernel<double>, dim3(blocks), dim3(threads), 0, c->stream,
+                                   static_cast<const double*>(c->tr.val), d_perm,
+                                   reinterpret_cast<double*>(c->d_tm_val), int64_t(ntm));
+            else
+                hipLaunchKernelGGL(gather_kernel<float>, dim3(blocks), dim3(threads), 0, c->stream,
+                                   static_cast<const float*>(c->tr.val), d_perm,
+                                   reinterpret_cast<float*>(c->d_tm_val), int64_t(ntm));
+            NOMAD_HIP_TRY(hipGetLastError());
+        } else {
+            NOMAD_HIP_TRY(hipMemsetAsync(c->d_tm_val, 0, ntm * size_t(c->rvsz), c->stream));
+        }
+        NOMAD_HIP_TRY(hipStreamSynchronize(c->stream));
+        dev_free(d_perm);
+    }
     NOMAD_HIP_TRY(hipMemcpy(c->d_tile_ptr, tptr.data(), tptr.size() * 8, hipMemcpyHostToDevice));
     NOMAD_HIP_TRY(hipMemcpy(c->d_tile_visits, vis.data(), vis.size() * sizeof(VisitDesc), hipMemcpyHostToDevice));
+    NOMAD_HIP_TRY(hipMemcpy(c->d_trow_ptr, trp.data(), trp.size() * 8, hipMemcpyHostToDevice));
+    if (!trows.empty())
+        NOMAD_HIP_TRY(hipMemcpy(c->d_trows, trows.data(), trows.size() * 4, hipMemcpyHostToDevice));
+    NOMAD_HIP_TRY(hipMemcpy(c->d_tslot, tm_slot.data(), ntm * 2, hipMemcpyHostToDevice));
+    NOMAD_HIP_TRY(hipMemcpy(c->d_tm_row, tm_row.data(), ntm * 4, hipMemcpyHostToDevice));
     c->tiles_ready = true;
     return NOMAD_OK;
 }
@@ -733,6 +839,11 @@ SgdArgs base_args(nomad_ctx* c) {
     if (c->p.mode == NOMAD_MODE_STRAT) {
         a.tile_ptr = c->d_tile_ptr;
         a.visits = c->d_tile_visits;
+        a.rowidx = c->d_tm_row;
+        a.val = c->d_tm_val;
+        a.trow_ptr = c->d_trow_ptr;
+        a.trows = c->d_trows;
+        a.tslot = c->d_tslot;
     }
     a.lrate = c->p.storage == NOMAD_STORE_FP64 ? c->p.lrate : double(float(c->p.lrate));
     a.drate = c->p.storage == NOMAD_STORE_FP64 ? c->p.drate : double(float(c->p.drate));

@@ -171,6 +171,15 @@ __device__ __forceinline__ void wait_all_mem() {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+// s_waitcnt vmcnt(0) alone (expcnt and lgkmcnt at their maxima, 7 and 15).
+// Placed right after loads whose registers a later shared loop reads, it
+// retires them there, so that the loop's own waits need not cover them.
+__device__ __forceinline__ void wait_vm() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 template <int STORE>
 struct Store;
 
@@ -428,6 +437,19 @@ __device__ __forceinline__ A wave_dot(A hx, A hy, A wx, A wy) {
 }
 
 __device__ __forceinline__ int readlane_i32(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+
+// A pointer into device memory as a global-address-space pointer.  Pointers
+// read from memory (rather than passed as kernel arguments) are generic, and
+// loads through them are flat loads; a flat load may complete out of order
+// with LDS accesses, so every wait the compiler places for one is vmcnt(0)
+// and lgkmcnt(0), which serialises loads issued ahead.  Loads through the
+// converted pointer are global loads with exact wait counts.
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> gbl(const T* p) {
+    return (gptr<T>)p;
+}
 __device__ __forceinline__ float readlane_f32(float x, int l) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
 }

@@ -71,7 +71,7 @@ struct SgdArgs {
     void* W;              // m x ldf, storage type
     void* H;              // n x ldf, storage type
     const int64_t* colptr;
-    const int32_t* rowidx;
+    const int32_t* rowidx;  // (strat: the ratings in tile order, indexed by visits[].start)
     const void* val;      // float (fp32/bf16 storage) or double (fp64)
     const void* steps;    // step table indexed by visit count: float or double
     double lrate, drate;  // for counts past the table
@@ -98,8 +98,11 @@ struct SgdArgs {
     uint64_t deadline;         // wall_clock64 ticks after its start at which a worker stops
 
     // strat schedule
-    const VisitDesc* visits;   // strat tile lists; .worker = column
+    const VisitDesc* visits;   // strat tile lists; .worker = column, .start in tile order
     const int64_t* tile_ptr;   // B * B + 1: first visit of tile (b, cb) at [b * B + cb] in visits
+    const int64_t* trow_ptr;   // B * B + 1: the tile's distinct rows at trows[trow_ptr[t] .. trow_ptr[t + 1])
+    const int32_t* trows;      //   ascending
+    const uint16_t* tslot;     // per rating (tile order): its row's rank among its tile's distinct rows
     int32_t substep;
     int32_t epoch;             // strat: the epoch; async: the launch index
 };

@@ -16,7 +16,7 @@ using nomad_k::VisitDesc;
 
 template <class A>
 __device__ __forceinline__ A step_for(const SgdArgs& a, int32_t count) {
-    if (count < a.nsteps) return static_cast<const A*>(a.steps)[count];
+    if (count < a.nsteps) return gbl(static_cast<const A*>(a.steps))[count];
     // past the table: same formula as the host table (nb_main.cpp:514-515)
     const double s = a.lrate * 1.5 / (1.0 + a.drate * pow(double(count) + 1.0, 1.5));
     return A(s);
@@ -158,8 +158,8 @@ __device__ __forceinline__ void run_visit_pre(const SgdArgs& a, const RowBlock& 
                                               typename Store<S>::acc reg, uint32_t count, uint32_t lkey) {
     using R = typename Store<S>::rval;
     const int lane = lane_id();
-    const int32_t* __restrict__ ridx = a.rowidx + start;
-    const R* __restrict__ rv = static_cast<const R*>(a.val) + start;
+    const gptr<int32_t> ridx = gbl(a.rowidx) + start;
+    const gptr<R> rv = gbl(static_cast<const R*>(a.val)) + start;
     const uint32_t sr_b = uint32_t(j) * 0x01000193u + count;
     run_chunk<S, DC, RM>(a, rb, h, row0, val0, min(kWave, len), step, reg, sr_b, lkey);
     for (int32_t base = kWave; base < len; base += kWave) {
@@ -177,6 +177,7 @@ __device__ __forceinline__ void run_visit_pre(const SgdArgs& a, const RowBlock& 
 // row stored.  Lanes past the row read zero and store nothing, as the
 // buffer accesses' range check does.
 constexpr int kTileLdsBytes = 8192;  // per wave
+constexpr int kStageRows = 256;      // most rows of W a tile stages in LDS
 
 template <int S>
 __device__ __forceinline__ typename Store<S>::raw lds_row_load(const typename Store<S>::raw* wl, int32_t off,
@@ -185,12 +186,12 @@ __device__ __forceinline__ typename Store<S>::raw lds_row_load(const typename St
     return inr ? wl[off] : z;
 }
 
-// run_chunk with the rows in LDS: the same steps in the same order (a
-// column's rows are distinct, so a row loaded ahead is never stale), the
-// stored forms written back to LDS.
+// run_chunk with the rows in LDS, rating t's row at slot myslot[t]: the same
+// steps in the same order (a column's rows are distinct, so a row loaded
+// ahead is never stale), the stored forms written back to LDS.
 template <int S, int RM>
-__device__ __forceinline__ void run_chunk_lds(const SgdArgs& a, typename Store<S>::raw* wl, int32_t rlo, int npairs,
-                                              typename Store<S>::pair& h, int32_t myrow,
+__device__ __forceinline__ void run_chunk_lds(const SgdArgs& a, typename Store<S>::raw* wl, int npairs,
+                                              typename Store<S>::pair& h, int32_t myrow, int32_t myslot,
                                               typename Store<S>::rval myval, int32_t cnt,
                                               typename Store<S>::acc step, typename Store<S>::acc reg,
                                               uint32_t sr_b, uint32_t lkey) {
@@ -203,7 +204,7 @@ __device__ __forceinline__ void run_chunk_lds(const SgdArgs& a, typename Store<S
     typename T::raw wp[kD];
 #pragma unroll
     for (int d = 0; d < kD; ++d)
-        wp[d] = lds_row_load<S>(wl, (readlane_i32(myrow, min(d, cnt - 1)) - rlo) * npairs + lane, inr);
+        wp[d] = lds_row_load<S>(wl, readlane_i32(myslot, min(d, cnt - 1)) * npairs + lane, inr);
     for (int32_t t0 = 0;; t0 += kD) {
 #pragma unroll
         for (int d = 0; d < kD; ++d) {
@@ -213,30 +214,35 @@ __device__ __forceinline__ void run_chunk_lds(const SgdArgs& a, typename Store<S
             P w = T::unpack(wp[d]);
             sgd_step<P, Aa>(w, h, Aa(readlane_val(myval, t)), step, reg);
             const uint32_t rnd = sr_bits_k(a.sr_seed, uint32_t(i), sr_b, lkey);
-            if (inr) wl[(i - rlo) * npairs + lane] = T::pack(w, rnd, RM);
-            wp[d] = lds_row_load<S>(wl, (readlane_i32(myrow, min(t + kD, cnt - 1)) - rlo) * npairs + lane, inr);
+            if (inr) wl[readlane_i32(myslot, t) * npairs + lane] = T::pack(w, rnd, RM);
+            wp[d] = lds_row_load<S>(wl, readlane_i32(myslot, min(t + kD, cnt - 1)) * npairs + lane, inr);
         }
     }
 chunk_done:;
 }
 
 template <int S, int RM>
-__device__ __forceinline__ void run_visit_lds(const SgdArgs& a, typename Store<S>::raw* wl, int32_t rlo, int npairs,
+__device__ __forceinline__ void run_visit_lds(const SgdArgs& a, typename Store<S>::raw* wl, int npairs,
                                               typename Store<S>::pair& h, int32_t j, int64_t start, int32_t len,
-                                              int32_t row0, typename Store<S>::rval val0,
+                                              int32_t row0, int32_t slot0, typename Store<S>::rval val0,
                                               typename Store<S>::acc step, typename Store<S>::acc reg,
                                               uint32_t count, uint32_t lkey) {
     using R = typename Store<S>::rval;
     const int lane = lane_id();
-    const int32_t* __restrict__ ridx = a.rowidx + start;
-    const R* __restrict__ rv = static_cast<const R*>(a.val) + start;
+    const gptr<int32_t> ridx = gbl(a.rowidx) + start;
+    const gptr<R> rv = gbl(static_cast<const R*>(a.val)) + start;
+    const gptr<uint16_t> sl = gbl(a.tslot) + start;
     const uint32_t sr_b = uint32_t(j) * 0x01000193u + count;
-    run_chunk_lds<S, RM>(a, wl, rlo, npairs, h, row0, val0, min(kWave, len), step, reg, sr_b, lkey);
+    run_chunk_lds<S, RM>(a, wl, npairs, h, row0, slot0, val0, min(kWave, len), step, reg, sr_b, lkey);
     for (int32_t base = kWave; base < len; base += kWave) {
         const int32_t cnt = min(kWave, len - base);
         const int32_t myrow = ridx[base + min(lane, cnt - 1)];
+        const int32_t myslot = sl[base + min(lane, cnt - 1)];
         const R myval = rv[base + min(lane, cnt - 1)];
-        run_chunk_lds<S, RM>(a, wl, rlo, npairs, h, myrow, myval, cnt, step, reg, sr_b, lkey);
+        // (the chunk loop is shared with the first chunk, whose ratings come
+        // from LDS: waiting here keeps its waits off the first chunk)
+        wait_vm();
+        run_chunk_lds<S, RM>(a, wl, npairs, h, myrow, myslot, myval, cnt, step, reg, sr_b, lkey);
     }
 }
 

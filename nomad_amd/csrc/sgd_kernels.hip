@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <type_traits>
 
 #include "sgd_common.hpp"
 
@@ -33,125 +34,263 @@ using nomad_k::VisitDesc;
 // the epoch as visit count (every (column, row block) pair is visited once per
 // epoch, as with the reference's per-thread col_update_counts at one rank).
 // The tile's non-empty column visits are listed ahead (tile_ptr / visits), so
-// empty (column, row block) pairs cost nothing.  While a visit runs, the next
-// two visits' h rows and first 64 rows / values are already being fetched:
-// their columns differ from the running one's (a column is visited once per
-// tile), so their h is not being written; their W rows may be, and are
-// loaded only when the visit starts.
+// empty (column, row block) pairs cost nothing.  A strat context keeps its
+// ratings in tile order (a tile's visits one after the other, each visit's
+// rows ascending), so that the ratings of consecutive visits are contiguous.
 // The single-problem kernel and the batched kernel both run a tile through
 // this function, so a problem's arithmetic is the same in either launch.
 //
-// When the tile's row block of W fits in the wave's LDS buffer (lds, with
-// room for lds_pairs stored pairs), the block is copied there at the start,
-// the tile's ratings read and write their rows there, and the block is copied
-// back at the end: the rows' values, and the order in which the steps see
-// them, are those of the HBM path, so the result is the same bit for bit,
-// but a step waits for LDS instead of HBM.
-template <int S, bool DC, int RM>
-__device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep, int epoch,
-                                           typename Store<S>::raw* lds, int lds_pairs) {
+// Visits run in groups of at most kK, each group as long as the first 64
+// ratings of each of its visits lie among the group's first 64 * kQ (the
+// first visit of a group always does).  A group's h rows and first 64 * kQ
+// ratings are staged in LDS one group ahead: at the start of group g the
+// registers holding group g + 1 (requested at the start of group g - 1) are
+// written to its LDS buffer and group g + 2 is requested into them, with its
+// descriptors requested one group earlier still.  Every wait for memory is
+// then for requests a whole group old, and no register is moved while its
+// load is in flight (a register copy of an in-flight load waits for it).  The
+// h rows may be read early: a column is visited once per tile, and the tiles
+// that wrote it earlier are complete; W rows are not read early.  The loop
+// is written out once for W in LDS and once for W in HBM, so that neither
+// holds the other's loads (a register that a load of the other path may
+// still be writing would cost a wait for all loads).
+//
+// When the rows of W that the tile touches (trows, at most kStageRows) fit
+// in the wave's LDS buffer, they are copied there at the start, packed in
+// ascending row order; each rating finds its row's place through tslot,
+// reads and writes it there, and the rows are copied back at the end: the
+// rows' values, and the order in which the steps see them, are those of the
+// HBM path, so the result is the same bit for bit, but a step waits for LDS
+// instead of HBM.
+template <int S>
+struct TileLds {
     using T = Store<S>;
+    using raw = typename T::raw;
+    using R = typename T::rval;
+    static constexpr int kW = kTileLdsBytes / int(sizeof(raw));  // staged W pairs
+    static constexpr int kK = S == NOMAD_STORE_FP64 ? 4 : 8;     // visits per group
+    static constexpr int kQ = S == NOMAD_STORE_BF16 ? 4 : (S == NOMAD_STORE_FP32 ? 2 : 1);  // x64 ratings
+    struct Group {
+        raw h[kK * kWave];      // visit u's h pair of lane l at [u * 64 + l]
+        int32_t key[kQ * kWave];  // rating's LDS slot (W staged) or row
+        R val[kQ * kWave];
+    };
+    raw w[kW];
+    Group g[2];
+};
+
+template <int S, bool DC, int RM>
+__device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep, int epoch, TileLds<S>& L) {
+    using T = Store<S>;
+    using TL = TileLds<S>;
     using E = typename T::elem;
     using Aa = typename T::acc;
     using R = typename T::rval;
+    using raw = typename T::raw;
+    constexpr int kK = TL::kK, kQ = TL::kQ;
     const int lane = lane_id();
     const int B = a.nblocks;
     const int cb = (b + substep) % B;
-    const int64_t* tp = a.tile_ptr + int64_t(b) * B + cb;
-    const int64_t v0 = tp[0], v1 = tp[1];
+    const int64_t tile = int64_t(b) * B + cb;
+    // (every load below goes through gbl(): the batched kernel reads its
+    // arguments from memory, so their pointers are generic)
+    const int64_t v0 = gbl(a.tile_ptr)[tile], v1 = gbl(a.tile_ptr)[tile + 1];
+    const int64_t tr0 = gbl(a.trow_ptr)[tile];
+    const int32_t nrows = int32_t(gbl(a.trow_ptr)[tile + 1] - tr0);
     if (v0 >= v1) return;
     const Aa step = step_for<Aa>(a, epoch);
     const Aa reg = Aa(a.reg);
     E* __restrict__ H = static_cast<E*>(a.H);
-    const R* __restrict__ rv = static_cast<const R*>(a.val);
+    const gptr<R> rv = gbl(static_cast<const R*>(a.val));
     const int npairs = a.ldf >> 1;
     const int32_t rlo = b * a.rpb, rhi = min(a.m, (b + 1) * a.rpb);
     const RowBlock rb = row_block<S>(a, rlo, rhi, lane);
     const uint32_t lkey = sr_lane_key(a.sr_seed, uint32_t(lane));
-    const int32_t block_pairs = (rhi - rlo) * npairs;
-    const bool in_lds = lds != nullptr && block_pairs <= lds_pairs;
+    const int32_t staged = nrows * npairs;
+    const bool in_lds = nrows <= kStageRows && staged <= TL::kW;
     constexpr int CP = DC ? kCpDevice : 0;
-    if (in_lds) {
-        // the block's rows are contiguous in W: block_pairs stored pairs,
-        // eight per lane in flight
-        constexpr int kU = 8;
-        for (int32_t q0 = 0; q0 < block_pairs; q0 += kU * kWave) {
-            typename T::raw t[kU];
-#pragma unroll
-            for (int u = 0; u < kU; ++u) t[u] = T::template load_raw_b<CP>(rb.rsrc, (q0 + u * kWave + lane) * T::kPairBytes);
-#pragma unroll
-            for (int u = 0; u < kU; ++u)
-                if (q0 + u * kWave + lane < block_pairs) lds[q0 + u * kWave + lane] = t[u];
-        }
-    }
-    // Visit descriptors 64 at a time, one per lane (the current batch and
-    // the next), so that no descriptor costs a memory round trip inside the
-    // loop; a visit's column factor and its first 64 rows and values are
-    // requested two visits ahead.  Indexes past the tile are clamped to v1
-    // (the next tile's first visit or the list's sentinel, in bounds) and
-    // what they load is dropped.
-    struct Descs {
+
+    // Group descriptors: lane u <= kK holds visit f + u (clamped to v1, the
+    // next tile's first visit or the list's sentinel: in bounds, and its
+    // start is this tile's end of ratings).
+    struct GDesc {
         int64_t start;
         int32_t len, col;
     };
-    auto load_descs = [&](int64_t vb) {
-        const VisitDesc dd = a.visits[min(vb + lane, v1)];
-        return Descs{dd.start, dd.len, dd.worker};
+    auto load_gdesc = [&](int64_t f) {
+        const gptr<VisitDesc> dd = gbl(a.visits) + min(f + min(lane, kK), v1);
+        return GDesc{dd->start, dd->len, dd->worker};
     };
-    int64_t vb = v0;
-    Descs dA = load_descs(vb), dB = load_descs(vb + kWave);
-    auto desc = [&](int64_t v, int64_t& st, int32_t& ln, int32_t& col) {
-        const int u = int(min(v, v1) - vb);
-        const Descs& q = u < kWave ? dA : dB;
-        const int l = u & (kWave - 1);
-        st = int64_t((uint64_t(uint32_t(readlane_i32(int32_t(uint64_t(q.start) >> 32), l))) << 32) |
-                     uint32_t(readlane_i32(int32_t(q.start), l)));
-        ln = readlane_i32(q.len, l);
-        col = readlane_i32(q.col, l);
+    auto start_of = [](const GDesc& d, int u) {
+        return int64_t((uint64_t(uint32_t(readlane_i32(int32_t(uint64_t(d.start) >> 32), u))) << 32) |
+                       uint32_t(readlane_i32(int32_t(d.start), u)));
     };
-    struct Pre {
-        typename T::pair h;
-        int32_t row;
-        R val;
+    // visits in the group that starts at f (d: its descriptors)
+    auto group_len = [&](const GDesc& d, int64_t f) {
+        const int64_t gs = start_of(d, 0);
+        const bool ok = lane < kK && d.start - gs + min(d.len, kWave) <= kQ * kWave;
+        const int nv = int(__builtin_ctzll(~__ballot(ok)));
+        return int(min(int64_t(nv), v1 - f));
     };
-    auto prefetch = [&](int64_t v) {
-        int64_t st;
-        int32_t ln, col;
-        desc(v, st, ln, col);
-        Pre p;
-        p.h = row_load<S, DC>(H + int64_t(col) * a.ldf, lane, npairs);
-        const int64_t fi = st + min(lane, max(ln, 1) - 1);
-        p.row = a.rowidx[fi];
-        p.val = rv[fi];
-        return p;
+
+    // Stored pair q of the LDS copy of W is pair q % npairs of the tile's row
+    // q / npairs; the row ids sit one per lane in trw[] (row s in trw[s / 64]
+    // of lane s % 64) and reach the lane that needs them by a lane permute.
+    // (q + 0.5) / npairs in fp32 rounds to within 2^-20 of the exact
+    // quotient, whose distance to the next integer is at least 0.5 / npairs.
+    int32_t trw[kStageRows / kWave];
+    const float inv_np = 1.0f / float(npairs);
+    auto row_of_slot = [&](int32_t sq) {
+        int32_t r = __shfl(trw[0], sq & (kWave - 1));
+#pragma unroll
+        for (int g = 1; g < kStageRows / kWave; ++g)
+            if (nrows > g * kWave) {
+                const int32_t rg = __shfl(trw[g], sq & (kWave - 1));
+                r = (sq >> 6) == g ? rg : r;
+            }
+        return r;
     };
-    Pre c0 = prefetch(v0), c1 = prefetch(v0 + 1);
-    for (int64_t v = v0; v < v1; ++v) {
-        if (v - vb == kWave) {
-            vb += kWave;
-            dA = dB;
-            dB = load_descs(vb + kWave);
-        }
-        const Pre c2 = prefetch(v + 2);
-        int64_t st;
-        int32_t ln, j;
-        desc(v, st, ln, j);
-        typename T::pair h = c0.h;
-        if (in_lds)
-            run_visit_lds<S, RM>(a, lds, rlo, npairs, h, j, st, ln, c0.row, c0.val, step, reg, uint32_t(epoch), lkey);
-        else
-            run_visit_pre<S, DC, RM>(a, rb, h, j, st, ln, c0.row, c0.val, step, reg, uint32_t(epoch), lkey);
-        const uint32_t rnd = sr_bits_k(a.sr_seed, 0x80000000u | uint32_t(b),
-                                       uint32_t(j) * 0x01000193u + uint32_t(epoch), lkey);
-        row_store<S, DC>(H + int64_t(j) * a.ldf, lane, npairs, h, rnd, RM);
-        c0 = c1;
-        c1 = c2;
-    }
+    auto staged_row = [&](int32_t q, int32_t& pr) {
+        const int32_t sq = min(int32_t((float(q) + 0.5f) * inv_np), nrows - 1);
+        pr = q - sq * npairs;
+        return row_of_slot(sq);
+    };
+
+    // the first three groups' descriptors from one batch of 64 (a group has
+    // at most kK visits, so the third ends before visit 3 * kK < 64)
+    const GDesc d64 = [&] {
+        const gptr<VisitDesc> dd = gbl(a.visits) + min(v0 + lane, v1);
+        return GDesc{dd->start, dd->len, dd->worker};
+    }();
     if (in_lds) {
-        // the block back to W (the range check drops the lanes past it)
-        for (int32_t q = lane; q < block_pairs + kWave - 1; q += kWave) {
-            const int32_t qq = min(q, block_pairs - 1);
-            T::template store_raw_b<CP>(rb.rsrc, q < block_pairs ? qq * T::kPairBytes : rb.drop(), lds[qq]);
+#pragma unroll
+        for (int g = 0; g < kStageRows / kWave; ++g) trw[g] = gbl(a.trows)[tr0 + min(g * kWave + lane, nrows - 1)];
+    }
+    // d64 seen from visit f: lane u gets d64's lane f - v0 + u (< 64, as
+    // f - v0 <= 2 kK), which holds visit min(f + u, v1)
+    auto shift = [&](int64_t f) {
+        const int src = int(f - v0) + min(lane, kK);
+        GDesc d;
+        d.start = int64_t((uint64_t(uint32_t(__shfl(int32_t(uint64_t(d64.start) >> 32), src))) << 32) |
+                          uint32_t(__shfl(int32_t(d64.start), src)));
+        d.len = __shfl(d64.len, src);
+        d.col = __shfl(d64.col, src);
+        return d;
+    };
+
+    auto run = [&](auto w_in_lds) {
+        constexpr bool LW = decltype(w_in_lds)::value;
+        // staging registers of one group: kK h pairs, kQ ratings per lane
+        struct Stage {
+            raw h[kK];
+            int32_t key[kQ];
+            R val[kQ];
+        };
+        auto request = [&](const GDesc& d, int nv, Stage& sg) {
+#pragma unroll
+            for (int u = 0; u < kK; ++u) {
+                const E* row = H + int64_t(readlane_i32(d.col, min(u, nv - 1))) * a.ldf;
+                sg.h[u] = T::template load_raw_b<CP>(row_rsrc(row, npairs * T::kPairBytes), lane * T::kPairBytes);
+            }
+            const int64_t gs = start_of(d, 0);
+            const int32_t n = int32_t(min(start_of(d, nv) - gs, int64_t(kQ * kWave)));
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                const int64_t e = gs + min(q * kWave + lane, max(n - 1, 0));
+                sg.key[q] = LW ? int32_t(gbl(a.tslot)[e]) : gbl(a.rowidx)[e];
+                sg.val[q] = rv[e];
+            }
+        };
+        auto publish = [&](const Stage& sg, typename TL::Group& G) {
+#pragma unroll
+            for (int u = 0; u < kK; ++u) G.h[u * kWave + lane] = sg.h[u];
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                G.key[q * kWave + lane] = sg.key[q];
+                G.val[q * kWave + lane] = sg.val[q];
+            }
+        };
+        if constexpr (LW) {
+            // every staged pair in one round of loads (kW <= kU * 64)
+            constexpr int kU = TL::kW / kWave;
+            raw t[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                if (u * kWave >= staged) break;
+                const int32_t q = u * kWave + lane;
+                int32_t pr;
+                const int32_t r = staged_row(q, pr);
+                t[u] = T::template load_raw_b<CP>(rb.rsrc, q < staged ? rb.soff(r) + pr * T::kPairBytes : rb.drop());
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                if (u * kWave >= staged) break;
+                if (u * kWave + lane < staged) L.w[u * kWave + lane] = t[u];
+            }
+        }
+        // groups g (dc, from fc, nc visits), g + 1 (d1, f1, n1), g + 2 (d2, f2)
+        int64_t fc = v0;
+        GDesc dc = shift(fc);
+        int nc = group_len(dc, fc);
+        int64_t f1 = fc + nc;
+        GDesc d1 = shift(f1);
+        int n1 = f1 < v1 ? group_len(d1, f1) : 1;
+        int64_t f2 = f1 + n1;
+        GDesc d2 = shift(f2);
+        Stage sg;
+        request(dc, nc, sg);
+        publish(sg, L.g[0]);
+        request(d1, n1, sg);
+        for (int p = 0; fc < v1; p ^= 1) {
+            // group g + 1 to LDS, group g + 2 requested, descriptors of g + 3
+            publish(sg, L.g[p ^ 1]);
+            const int n2 = f2 < v1 ? group_len(d2, f2) : 1;
+            request(d2, n2, sg);
+            const int64_t f3 = f2 + n2;
+            const GDesc d3 = load_gdesc(f3);
+            const typename TL::Group& G = L.g[p];
+            const int64_t gs = start_of(dc, 0);
+            for (int u = 0; u < nc; ++u) {
+                const int64_t st = start_of(dc, u);
+                const int32_t ln = readlane_i32(dc.len, u), j = readlane_i32(dc.col, u);
+                typename T::pair h = T::unpack(G.h[u * kWave + lane]);
+                // the visit's first 64 ratings are staged (the group's rule)
+                const int32_t o = int32_t(st - gs) + min(lane, ln - 1);
+                const int32_t key = G.key[o];
+                const R val = G.val[o];
+                if constexpr (LW) {
+                    const int32_t row = RM == NOMAD_ROUND_STOCHASTIC ? row_of_slot(key) : 0;
+                    run_visit_lds<S, RM>(a, L.w, npairs, h, j, st, ln, row, key, val, step, reg, uint32_t(epoch),
+                                         lkey);
+                } else {
+                    run_visit_pre<S, DC, RM>(a, rb, h, j, st, ln, key, val, step, reg, uint32_t(epoch), lkey);
+                }
+                const uint32_t rnd = sr_bits_k(a.sr_seed, 0x80000000u | uint32_t(b),
+                                               uint32_t(j) * 0x01000193u + uint32_t(epoch), lkey);
+                row_store<S, DC>(H + int64_t(j) * a.ldf, lane, npairs, h, rnd, RM);
+            }
+            fc = f1;
+            nc = n1;
+            dc = d1;
+            f1 = f2;
+            n1 = n2;
+            d1 = d2;
+            f2 = f3;
+            d2 = d3;
+        }
+    };
+    if (in_lds)
+        run(std::true_type{});
+    else
+        run(std::false_type{});
+    if (in_lds) {
+        // the rows back to W (the range check drops the lanes past them)
+        for (int32_t q0 = 0; q0 < staged; q0 += kWave) {
+            const int32_t q = q0 + lane;
+            int32_t pr;
+            const int32_t r = staged_row(q, pr);
+            T::template store_raw_b<CP>(rb.rsrc, q < staged ? rb.soff(r) + pr * T::kPairBytes : rb.drop(),
+                                        L.w[min(q, staged - 1)]);
         }
     }
 }
@@ -160,12 +299,10 @@ __device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep,
 // branches on it; fp32 and fp64 ignore it)
 template <int S, int RM>
 __global__ __launch_bounds__(64) void sgd_strat_kernel(SgdArgs a) {
-    using raw = typename Store<S>::raw;
-    constexpr int kPairs = kTileLdsBytes / int(sizeof(raw));
-    __shared__ raw wl[kPairs];
+    __shared__ TileLds<S> tl;
     const int b = blockIdx.x;
     if (b >= a.nblocks) return;
-    strat_tile<S, false, RM>(a, b, a.substep, a.epoch, wl, kPairs);
+    strat_tile<S, false, RM>(a, b, a.substep, a.epoch, tl);
 }
 
 // ---------------------------------------------------------------------------
@@ -200,9 +337,7 @@ __global__ __launch_bounds__(256) void sgd_batch_kernel(BatchArgs ba) {
     __shared__ int32_t seg_w[kBatchMax];
     __shared__ int32_t boff[kBatchMax + 1];
     __shared__ int32_t order[kBatchMax];
-    using raw = typename Store<S>::raw;
-    constexpr int kPairs = kTileLdsBytes / int(sizeof(raw));
-    __shared__ raw wl[4][kPairs];  // per wave (256-thread workgroups)
+    __shared__ TileLds<S> tl[4];  // per wave (256-thread workgroups)
     const int Q = ba.nsorted, G = ba.nseg;
     for (int i = threadIdx.x; i <= Q; i += blockDim.x) {
         if (i < G) {
@@ -256,11 +391,11 @@ __global__ __launch_bounds__(256) void sgd_batch_kernel(BatchArgs ba) {
         }
         if constexpr (S == NOMAD_STORE_BF16) {  // problems may round differently
             if (a.round_mode == NOMAD_ROUND_STOCHASTIC)
-                strat_tile<S, true, NOMAD_ROUND_STOCHASTIC>(a, b, r % B, a.epoch + r / B, wl[wave], kPairs);
+                strat_tile<S, true, NOMAD_ROUND_STOCHASTIC>(a, b, r % B, a.epoch + r / B, tl[wave]);
             else
-                strat_tile<S, true, NOMAD_ROUND_NEAREST>(a, b, r % B, a.epoch + r / B, wl[wave], kPairs);
+                strat_tile<S, true, NOMAD_ROUND_NEAREST>(a, b, r % B, a.epoch + r / B, tl[wave]);
         } else {
-            strat_tile<S, true, NOMAD_ROUND_NEAREST>(a, b, r % B, a.epoch + r / B, wl[wave], kPairs);
+            strat_tile<S, true, NOMAD_ROUND_NEAREST>(a, b, r % B, a.epoch + r / B, tl[wave]);
         }
         wait_all_mem();
         if (lane == 0) st_dc(done + b, uint32_t(r + 1));
