@@ -145,12 +145,19 @@ def profiled_traffic(storage: str) -> float | None:
     """HBM (L2-to-fabric) bytes per update of the async kernel measured by the
     two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE, each scaled by a
     plain copy of known size) summarised in profiles/; None if absent."""
+    return profile_figure("sgd_async_kernel_netflix_k100_pmc.md", f"traffic_bytes_per_update[{storage}]")
+
+
+def profile_figure(name: str, key: str) -> float | None:
+    """A `key: value` line of profiles/<name> (copy-calibrated counter
+    figures written from a rocprofv3 run of the same workload); None if the
+    file or the line is absent."""
     import re
 
-    f = ROOT / "profiles" / "sgd_async_kernel_netflix_k100_pmc.md"
+    f = ROOT / "profiles" / name
     if not f.exists():
         return None
-    m = re.search(rf"^traffic_bytes_per_update\[{storage}\]: (\S+)$", f.read_text(), flags=re.M)
+    m = re.search(rf"^{re.escape(key)}: (\S+)$", f.read_text(), flags=re.M)
     return float(m.group(1)) if m else None
 
 
@@ -277,6 +284,8 @@ def sweep(a, lib) -> dict:
     for c in batch + seq:
         c.close()
     achieved = alg / dev / 1e9
+    tb = profile_figure("sgd_batch_kernel_sweep32.md", "traffic_bytes_per_update[sweep]")
+    traffic = tb * upd / a.steps if tb else None
     return {
         "metric": "batched sweep updates/s, 32 uneven problems, BF16 storage, 1 GPU",
         "value": upd / dev,
@@ -297,7 +306,10 @@ def sweep(a, lib) -> dict:
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "sgd_batch_kernel<bf16>",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "sgd_batch_kernel<bf16>",
+            "frac_measured_traffic": traffic / (dev / a.steps) / 1e9 / HBM_PEAK_GBS if traffic else None,
+            "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per update (copy-calibrated), "
+                              "profiles/sgd_batch_kernel_sweep32.md, x updates per launch",
             "alg_bytes_per_launch": alg / a.steps, "avg_launch_ms": dev / a.steps * 1e3,
         },
         "sequential": {"updates_per_s": upd / seq_dev, "ms_per_step": seq_dev / a.steps * 1e3,
@@ -320,15 +332,20 @@ def ccd_workload(a, lib) -> dict:
     log(f"ccd: {st.ranks_done} ranks, {st.inner_done} inner iterations in {st.solve_seconds:.3f} s "
         f"({st.seconds:.3f} s with host set-up), final obj {obj[-1]:.6g} test RMSE {rmse[-1]:.6f}")
     gbs = st.alg_bytes / st.solve_seconds / 1e9
+    pr = profile_figure("ccd_pass_netflix_k20.md", "traffic_bytes_per_rank[ccd]")
+    pi = profile_figure("ccd_pass_netflix_k20.md", "traffic_bytes_per_inner[ccd]")
+    traffic = st.ranks_done * pr + st.inner_done * pi if (pr and pi and a.scale == 1.0) else None
     return {"metric": "CCD++ rank-one sweep, Netflix-shape, FP64, 1 GPU", "value": st.inner_done / st.solve_seconds,
             "unit": "inner iterations/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
             "data": "synthetic", "seconds": st.solve_seconds, "seconds_with_setup": st.seconds,
             "config": {"workload": f"ccd++ k={k}, 1 outer iteration, {a.ccd_inner} inner, lambda 0.05",
                        "m": tr.m, "n": tr.n, "nnz_train": tr.nnz},
             "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": gbs / HBM_PEAK_GBS, "traffic": None,
-                         "note": "wall time of the rank / inner-iteration loop (host early-stop reads "
-                                 "included, host set-up excluded)"},
+                         "frac": gbs / HBM_PEAK_GBS, "traffic": traffic,
+                         "frac_measured_traffic": traffic / st.solve_seconds / 1e9 / HBM_PEAK_GBS if traffic else None,
+                         "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per rank and per inner iteration "
+                                           "(copy-calibrated), profiles/ccd_pass_netflix_k20.md",
+                         "note": "wall time of the rank / inner-iteration loop (host set-up excluded)"},
             "final_test_rmse": float(rmse[-1])}
 
 
@@ -369,13 +386,19 @@ def objective_workload(a, lib) -> dict:
         out[storage] = {"ms_per_eval": dt * 1e3, "loss": L, "alg_GBps": alg / dt / 1e9}
         log(f"objective {storage}: {dt * 1e3:.3f} ms per evaluation (loss {L:.6g})")
     gbs = out["fp64"]["alg_GBps"]
+    tb = profile_figure("obj_pass_100k_20k_10m_k64.md", "traffic_bytes_per_eval[fp64]")
+    traffic = tb if (tb and a.scale == 1.0) else None
     return {"metric": "objective + gradient evaluations/s, 100k x 20k x 10M, k=64, FP64, 1 GPU",
             "value": 1e3 / out["fp64"]["ms_per_eval"], "unit": "evaluations/s", "n_gpus": 1,
             "higher_is_better": True, "dtype": "f64", "data": "synthetic", "steps": a.steps,
             "config": {"workload": "objective and full gradient, device arrays", "m": tr.m, "n": tr.n,
                        "nnz": tr.nnz, "k": k},
             "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                         "frac": gbs / HBM_PEAK_GBS, "traffic": traffic,
+                         "frac_measured_traffic": (traffic / (out["fp64"]["ms_per_eval"] * 1e-3) / 1e9 /
+                                                   HBM_PEAK_GBS) if traffic else None,
+                         "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per evaluation (copy-calibrated), "
+                                           "profiles/obj_pass_100k_20k_10m_k64.md",
                          "note": "host wall time per evaluation (launches + one loss read back); the gathered "
                                  "rows (W: 51 MB, H: 10 MB in FP64) are served largely by the L2 and the "
                                  "memory-side cache"},
