@@ -66,7 +66,9 @@ extern "C" {
 typedef struct nomad_ctx nomad_ctx;
 
 typedef struct nomad_params {
-    int32_t m, n, k;   /* rows (users), columns (items), latent dimension (--dim) */
+    int32_t m, n, k;   /* rows (users), columns (items), latent dimension (--dim,
+                          1..256; above 128 the kernels hold two element pairs
+                          per lane)                                               */
     int32_t storage;   /* NOMAD_STORE_*                                          */
     int32_t mode;      /* NOMAD_MODE_*                                           */
     int32_t rounding;  /* NOMAD_ROUND_* (bf16 only)                              */
@@ -77,7 +79,8 @@ typedef struct nomad_params {
                           async mode holds at most this many columns resident
                           per workgroup (0: no cap).  The workgroup's LDS
                           slots bound it: 128 for fp32 and bf16 at 8 waves per
-                          workgroup, 64 at 4 or 16 waves, 32 for fp64       */
+                          workgroup, 64 at 4 or 16 waves, 32 for fp64 (k >
+                          128: 64 at 8 waves, 32 otherwise)                 */
     int32_t device;    /* HIP device ordinal                                     */
     int32_t flags;     /* NOMAD_FLAG_* bits; others must be 0                    */
     double lrate;      /* --lrate  */

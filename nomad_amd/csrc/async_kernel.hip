@@ -64,16 +64,18 @@ constexpr int kBatch = 16;       // inbox slots examined per stream
 // 64 KiB.
 template <int S, int NW>
 constexpr int stream_depth() {
-    return (NW == 8 && S != NOMAD_STORE_FP64) ? 4
-                                              : (64 * 1024) / (NW * kWave * 2 * int(sizeof(typename Store<S>::acc)));
+    // (32 KiB of rings in the two-pair layout, whose slots are larger)
+    return (NW == 8 && base_storage(S) != NOMAD_STORE_FP64)
+               ? 4
+               : (64 * 1024 / Store<S>::kNP) / (NW * kWave * 2 * int(sizeof(typename Store<S>::acc)) * Store<S>::kNP);
 }
 
 // W rows in flight inside one long piece: deep enough to cover HBM latency
 // with one wave per SIMD (4 waves per workgroup), less where more waves share
 // the SIMD (at 8 waves 8 rows were as fast as 16)
-template <int NW>
+template <int S, int NW>
 constexpr int long_prefetch_depth() {
-    return NW <= 4 ? 32 : 8;
+    return (NW <= 4 ? 32 : 8) / Store<S>::kNP;
 }
 
 template <int S, int NW, int SLOTS>
@@ -84,7 +86,7 @@ struct AsyncShared {
     static constexpr int kVd = stream_depth<S, NW>();
     typename Store<S>::raw vic[NW][kVd][kWave];  // per wave: the last kVd updated rows of its stream,
                                                  // as stored
-    A h[SLOTS][2 * kWave];       // column factors of resident tokens
+    A h[SLOTS][2 * kWave * Store<S>::kNP];  // column factors of resident tokens
     int32_t wrow[SLOTS][kWave];  // window of the column's ratings here: rows
     R wval[SLOTS][kWave];        //                                    values
     int32_t col[SLOTS];
@@ -179,7 +181,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
     static_assert(SLOTS % 32 == 0 && (SLOTS & (SLOTS - 1)) == 0, "SLOTS must be a power of two >= 32");
     static_assert(NW <= 16 && kBatch <= kWave, "layout limits");
     constexpr int kPf = stream_depth<S, NW>();
-    static_assert(kWave % kPf == 0 && kWave % long_prefetch_depth<NW>() == 0, "depths must divide the wave");
+    static_assert(kWave % kPf == 0 && kWave % long_prefetch_depth<S, NW>() == 0, "depths must divide the wave");
     __shared__ AsyncShared<S, NW, SLOTS> sh;
     const int g = blockIdx.x;
     // wave-uniform values are made scalar so that branches on them stay
@@ -213,13 +215,12 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
     // within the block is the scalar offset of its load or store, always a
     // row of this wave; the hardware range check covers the per-lane offset
     // (not the scalar one), which is the lane's pair offset, 2^31 for lanes
-    // past the row, or kDrop with a scalar offset of 0 for a store to drop
+    // past the row, or drop_voff with a scalar offset of 0 for a store to drop
     // (out-of-range loads read zero, stores are dropped).
     const int rowbytes = a.ldf * int(sizeof(E));
     const __amdgpu_buffer_rsrc_t wrs = __builtin_amdgcn_make_buffer_rsrc(
         W + int64_t(row_lo) * ldf, 0, (row_hi - row_lo) * rowbytes, 0x00020000);
-    const uint32_t wvoff = lane < npairs ? uint32_t(lane * T::kPairBytes) : 0x80000000u;
-    const int kDrop = int(wvoff + 0x7FFFFFFFu);  // every lane out of range (also kDrop - d * 1024, d < 64)
+    const typename T::voff_t wvoff = T::lane_voff(lane, npairs);
     const Aa sel63 = lane == kWave - 1 ? Aa(1) : Aa(0);  // where a rating enters the dot product's tree
 
     for (int i = threadIdx.x; i < NW * SLOTS; i += blockDim.x) (&sh.inbox[0][0])[i] = 0u;
@@ -325,8 +326,8 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                 for (int d = 0; d < kPf; ++d) {
                     soff[d] = readlane_i32(soffv, min(d, total - 1));
                     // (distinct offsets: stores to one offset would be merged away)
-                    T::store_b(wrs, kDrop - d * 1024, 0, P{Aa(0), Aa(0)}, 0u, RM);
-                    wp[d] = T::load_raw_b(wrs, int(wvoff), soff[d]);
+                    T::store_b(wrs, T::drop_voff(wvoff, d), 0, T::zero(), 0u, RM);
+                    wp[d] = T::load_raw_b(wrs, wvoff, soff[d]);
                 }
                 Aa t_step = Aa(0);
                 uint32_t t_srb = 0;
@@ -360,8 +361,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                 int cur_t = 0;
                 int s_cur = readlane_i32(ts, 0);
                 P h;
-                h.x = sh.h[s_cur][2 * lane];
-                h.y = sh.h[s_cur][2 * lane + 1];
+                h = T::lds_get(sh.h[s_cur], lane);
                 // per piece: c1 = 1 - step * reg and -step (the step's two coefficients)
                 const Aa t_c1 = Aa(1) - t_step * reg;
                 Aa c1 = readlane_val(t_c1, 0);
@@ -381,12 +381,10 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                         if (d >= left) goto stream_done;
                         const int u = u0 + d;
                         if ((pm >> d) & 1u) {  // next piece: park h, take the next column's
-                            sh.h[s_cur][2 * lane] = h.x;
-                            sh.h[s_cur][2 * lane + 1] = h.y;
+                            T::lds_put(sh.h[s_cur], lane, h);
                             ++cur_t;
                             s_cur = readlane_i32(ts, cur_t);
-                            h.x = sh.h[s_cur][2 * lane];
-                            h.y = sh.h[s_cur][2 * lane + 1];
+                            h = T::lds_get(sh.h[s_cur], lane);
                             c1 = readlane_val(t_c1, cur_t);
                             nstep = -__shfl(t_step, cur_t);
                         }
@@ -394,25 +392,23 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                         // of updated rows
                         P w = T::unpack(wp[d]);
                         const int pd = readlane_i32(pdist, u);
-                        P fw = P{Aa(0), Aa(0)};
+                        P fw = T::zero();
                         if (pd != 0) fw = T::unpack(sh.vic[wave][(d - pd + kPf) % kPf][lane]);
-                        w.x = pd != 0 ? fw.x : w.x;
-                        w.y = pd != 0 ? fw.y : w.y;
-                        sgd_step_b<P, Aa>(w, h, Aa(readlane_val(sval, u)), sel63, c1, nstep);
+                        w = T::select(pd != 0, fw, w);
+                        sgd_step_b(w, h, Aa(readlane_val(sval, u)), sel63, c1, nstep);
                         sr_acc += 0x9E3779B9u;
                         const typename T::raw wr = T::pack(w, lkey + sr_acc, RM);
                         // (the offset is uniform; for bf16 the compiler carries some
                         // soff[d] across the loop in a vector register, and without
                         // the readfirstlane it wraps the store in a per-value loop)
-                        T::store_raw_b(wrs, int(wvoff), wr, __builtin_amdgcn_readfirstlane(soff[d]));
+                        T::store_raw_b(wrs, wvoff, wr, __builtin_amdgcn_readfirstlane(soff[d]));
                         sh.vic[wave][d][lane] = wr;
                         soff[d] = readlane_i32(soffN, u);
-                        wp[d] = T::load_raw_b(wrs, int(wvoff), soff[d]);
+                        wp[d] = T::load_raw_b(wrs, wvoff, soff[d]);
                     }
                 }
             stream_done:
-                sh.h[s_cur][2 * lane] = h.x;
-                sh.h[s_cur][2 * lane + 1] = h.y;
+                T::lds_put(sh.h[s_cur], lane, h);
                 tk_chunk += int64_t(clk() - t_c0);
 
                 // ---- hand each slot on: next wave here, or out of the workgroup
@@ -454,9 +450,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                         const int t = int(__builtin_ctzll(mm));
                         const int s = readlane_i32(ts, t);
                         const int32_t j = readlane_i32(t_j, t);
-                        P hx;
-                        hx.x = sh.h[s][2 * lane];
-                        hx.y = sh.h[s][2 * lane + 1];
+                        const P hx = T::lds_get(sh.h[s], lane);
                         const uint32_t rnd = sr_bits_k(a.sr_seed, 0x80000000u | uint32_t(g),
                                                      uint32_t(readlane_i32(int32_t(t_srb), t)), lkey);
                         T::store_dc(H + int64_t(j) * ldf, lane, npairs, hx, rnd, RM);
@@ -506,14 +500,12 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
             const Aa step = sh.stp[s];
             const uint32_t sr_b = uint32_t(j) * 0x01000193u + uint32_t(cycle);
             const Aa c1 = Aa(1) - step * reg;
-            P h;
-            h.x = sh.h[s][2 * lane];
-            h.y = sh.h[s][2 * lane + 1];
+            P h = T::lds_get(sh.h[s], lane);
             // Stream through the column's ratings from cur while the rows are
             // this wave's: two 64-rating chunks of the CSC range in registers,
             // W rows kPfL ratings ahead across the chunk boundary.  Rows of
             // one column are distinct, so no update needs forwarding.
-            constexpr int kPfL = long_prefetch_depth<NW>();
+            constexpr int kPfL = long_prefetch_depth<S, NW>();
             const int64_t cbase = int64_t(vstart);
             int32_t c0 = cur;  // range offset of chunk A
             // the 64 ratings from range offset o (rows past the range: INT_MAX).
@@ -552,8 +544,8 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
 #pragma unroll
                 for (int d = 0; d < kPfL; ++d) {
                     soff[d] = readlane_i32(offA, d);
-                    T::store_b(wrs, kDrop - d * 1024, 0, P{Aa(0), Aa(0)}, 0u, RM);  // dropped: see the stream prologue
-                    wp[d] = T::load_raw_b(wrs, int(wvoff), soff[d]);
+                    T::store_b(wrs, T::drop_voff(wvoff, d), 0, T::zero(), 0u, RM);  // dropped: see the stream prologue
+                    wp[d] = T::load_raw_b(wrs, wvoff, soff[d]);
                 }
             }
             const Aa nstep = -step;
@@ -564,11 +556,11 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
             // cntA mod kPfL ratings follow the chunk loop
             auto rating = [&](int uu, int d) {
                 P w = T::unpack(wp[d]);
-                sgd_step_b<P, Aa>(w, h, Aa(readlane_val(vA, uu)), sel63, c1, nstep);
+                sgd_step_b(w, h, Aa(readlane_val(vA, uu)), sel63, c1, nstep);
                 sr_acc += 0x9E3779B9u;
-                T::store_raw_b(wrs, int(wvoff), T::pack(w, lkey + sr_acc, RM), soff[d]);
+                T::store_raw_b(wrs, wvoff, T::pack(w, lkey + sr_acc, RM), soff[d]);
                 soff[d] = readlane_i32(offN, uu);
-                wp[d] = T::load_raw_b(wrs, int(wvoff), soff[d]);
+                wp[d] = T::load_raw_b(wrs, wvoff, soff[d]);
             };
             for (;;) {
                 const int full = cntA & ~(kPfL - 1);
@@ -622,8 +614,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
             tk_visit += int64_t(t_v1 - t_v0);
             if (next_row >= 0) {
                 // more of this column lives in a later row block of this workgroup
-                sh.h[s][2 * lane] = h.x;
-                sh.h[s][2 * lane + 1] = h.y;
+                T::lds_put(sh.h[s], lane, h);
                 if (lane == 0) {
                     sh.cur[s] = cur;
                     sh.wb[s] = wb;
@@ -813,8 +804,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
                     if (t < nb) {
                         const int s = readlane_i32(myslot, t);
                         const int32_t ln = readlane_i32(tlen, t);
-                        sh.h[s][2 * lane] = hv[u].x;
-                        sh.h[s][2 * lane + 1] = hv[u].y;
+                        T::lds_put(sh.h[s], lane, hv[u]);
                         sh.wrow[s][lane] = wr[u];
                         sh.wval[s][lane] = wv[u];
                         write_bounds<NW>(sh, s, wr[u], min(kWave, ln), lane_bound);
@@ -861,9 +851,7 @@ __global__ __launch_bounds__(NW * 64) void sgd_async_kernel(SgdArgs a) {
     for (int s = wave; s < SLOTS; s += NW) {
         if (lds_ld(&sh.freebits[s >> 5]) & (1u << (s & 31))) continue;
         const int32_t j = sh.col[s];
-        P h;
-        h.x = sh.h[s][2 * lane];
-        h.y = sh.h[s][2 * lane + 1];
+        const P h = T::lds_get(sh.h[s], lane);
         const uint32_t rnd = sr_bits_k(a.sr_seed, 0x80000000u | uint32_t(g),
                                      uint32_t(j) * 0x01000193u + uint32_t(sh.cyc[s]), lkey);
         T::store_dc(H + int64_t(j) * ldf, lane, npairs, h, rnd, RM);
@@ -900,15 +888,19 @@ __global__ void seed_rings_kernel(SgdArgs a, const int32_t* list, const int64_t*
 
 // The rounding of bf16 stores is a template argument, so the per-rating
 // store carries no branch on it; fp32 and fp64 ignore it.
-#define NOMAD_ASYNC_INST(NW)                                                                                      \
-    template __global__ void                                                                                      \
-    sgd_async_kernel<NOMAD_STORE_FP32, NW, async_slots<NOMAD_STORE_FP32, NW>(), NOMAD_ROUND_NEAREST>(SgdArgs);    \
-    template __global__ void                                                                                      \
-    sgd_async_kernel<NOMAD_STORE_BF16, NW, async_slots<NOMAD_STORE_BF16, NW>(), NOMAD_ROUND_NEAREST>(SgdArgs);    \
-    template __global__ void                                                                                      \
-    sgd_async_kernel<NOMAD_STORE_BF16, NW, async_slots<NOMAD_STORE_BF16, NW>(), NOMAD_ROUND_STOCHASTIC>(SgdArgs); \
-    template __global__ void                                                                                      \
-    sgd_async_kernel<NOMAD_STORE_FP64, NW, async_slots<NOMAD_STORE_FP64, NW>(), NOMAD_ROUND_NEAREST>(SgdArgs);
-NOMAD_ASYNC_INST(4)
-NOMAD_ASYNC_INST(8)
-NOMAD_ASYNC_INST(16)
+// (storage | kWideBit: rows of more than 128 elements, two pairs per lane)
+#define NOMAD_ASYNC_INST(NW, WB)                                                                          \
+    template __global__ void sgd_async_kernel<NOMAD_STORE_FP32 | WB, NW, async_slots<NOMAD_STORE_FP32 | WB, NW>(), \
+                                              NOMAD_ROUND_NEAREST>(SgdArgs);                                   \
+    template __global__ void sgd_async_kernel<NOMAD_STORE_BF16 | WB, NW, async_slots<NOMAD_STORE_BF16 | WB, NW>(), \
+                                              NOMAD_ROUND_NEAREST>(SgdArgs);                                   \
+    template __global__ void sgd_async_kernel<NOMAD_STORE_BF16 | WB, NW, async_slots<NOMAD_STORE_BF16 | WB, NW>(), \
+                                              NOMAD_ROUND_STOCHASTIC>(SgdArgs);                                \
+    template __global__ void sgd_async_kernel<NOMAD_STORE_FP64 | WB, NW, async_slots<NOMAD_STORE_FP64 | WB, NW>(), \
+                                              NOMAD_ROUND_NEAREST>(SgdArgs);
+NOMAD_ASYNC_INST(4, 0)
+NOMAD_ASYNC_INST(8, 0)
+NOMAD_ASYNC_INST(16, 0)
+NOMAD_ASYNC_INST(4, kWideBit)
+NOMAD_ASYNC_INST(8, kWideBit)
+NOMAD_ASYNC_INST(16, kWideBit)

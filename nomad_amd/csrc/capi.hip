@@ -853,7 +853,7 @@ SgdArgs base_args(nomad_ctx* c) {
 
 template <int S>
 hipError_t launch_strat(const SgdArgs& a, hipStream_t st) {
-    if constexpr (S == NOMAD_STORE_BF16) {
+    if constexpr ((S & 15) == NOMAD_STORE_BF16) {
         if (a.round_mode == NOMAD_ROUND_STOCHASTIC) {
             hipLaunchKernelGGL((sgd_strat_kernel<S, NOMAD_ROUND_STOCHASTIC>), dim3(a.nblocks), dim3(64), 0, st, a);
             return hipGetLastError();
@@ -897,7 +897,7 @@ hipError_t launch_async_rm(const SgdArgs& a, int nw, hipStream_t st) {
 
 template <int S>
 hipError_t launch_async(const SgdArgs& a, int nw, int rounding, hipStream_t st) {
-    if constexpr (S == NOMAD_STORE_BF16) {
+    if constexpr ((S & 15) == NOMAD_STORE_BF16) {
         if (rounding == NOMAD_ROUND_STOCHASTIC) return launch_async_rm<S, NOMAD_ROUND_STOCHASTIC>(a, nw, st);
     }
     return launch_async_rm<S, NOMAD_ROUND_NEAREST>(a, nw, st);
@@ -920,7 +920,7 @@ hipError_t async_occupancy_rm(int nw, int* per_cu) {
 
 template <int S>
 hipError_t async_occupancy(int nw, int rounding, int* per_cu) {
-    if constexpr (S == NOMAD_STORE_BF16) {
+    if constexpr ((S & 15) == NOMAD_STORE_BF16) {
         if (rounding == NOMAD_ROUND_STOCHASTIC) return async_occupancy_rm<S, NOMAD_ROUND_STOCHASTIC>(nw, per_cu);
     }
     return async_occupancy_rm<S, NOMAD_ROUND_NEAREST>(nw, per_cu);
@@ -933,12 +933,21 @@ hipError_t launch_sse(const SseArgs& a, int64_t waves, hipStream_t st) {
     return hipGetLastError();
 }
 
+// Kernel storage code: the storage type, plus kWideLayout when a factor row
+// has more than 128 elements (the kernels then hold two element pairs per
+// lane: pairs l and l + 64).
+constexpr int kWideLayout = 16;
+int kernel_code(int storage, int ldf) { return storage | (ldf > 128 ? kWideLayout : 0); }
+
 template <class F>
-hipError_t dispatch_storage(int storage, F&& f) {
-    switch (storage) {
+hipError_t dispatch_storage(int code, F&& f) {
+    switch (code) {
         case NOMAD_STORE_FP32: return f(std::integral_constant<int, NOMAD_STORE_FP32>{});
         case NOMAD_STORE_BF16: return f(std::integral_constant<int, NOMAD_STORE_BF16>{});
-        default: return f(std::integral_constant<int, NOMAD_STORE_FP64>{});
+        case NOMAD_STORE_FP64: return f(std::integral_constant<int, NOMAD_STORE_FP64>{});
+        case NOMAD_STORE_FP32 | kWideLayout: return f(std::integral_constant<int, NOMAD_STORE_FP32 | kWideLayout>{});
+        case NOMAD_STORE_BF16 | kWideLayout: return f(std::integral_constant<int, NOMAD_STORE_BF16 | kWideLayout>{});
+        default: return f(std::integral_constant<int, NOMAD_STORE_FP64 | kWideLayout>{});
     }
 }
 
@@ -1012,7 +1021,7 @@ int train_async(nomad_ctx* c, double seconds, int64_t epochs, nomad_train_stats*
     }
     if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
     if (e == hipSuccess)
-        e = dispatch_storage(c->p.storage,
+        e = dispatch_storage(kernel_code(c->p.storage, c->ldf),
                              [&](auto s) { return launch_async<decltype(s)::value>(a, c->nw, c->p.rounding, c->stream); });
     if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -1098,7 +1107,7 @@ int train_strat(nomad_ctx* c, int64_t epochs, nomad_train_stats* st) {
         a.epoch = int32_t(c->strat_epochs + ep);
         for (int s = 0; s < c->P; ++s) {
             a.substep = s;
-            hipError_t e = dispatch_storage(c->p.storage,
+            hipError_t e = dispatch_storage(kernel_code(c->p.storage, c->ldf),
                                             [&](auto t) { return launch_strat<decltype(t)::value>(a, c->stream); });
             if (e != hipSuccess) return fail(NOMAD_EHIP, std::string("strat launch: ") + hipGetErrorString(e));
         }
@@ -1122,8 +1131,10 @@ int train_batch(nomad_ctx* const* cs, int32_t count, int64_t epochs, int32_t wav
         const nomad_ctx* c = cs[i];
         if (!c) return fail(NOMAD_EINVAL, "nomad_train_batch: null context");
         if (c->p.mode != NOMAD_MODE_STRAT) return fail(NOMAD_EINVAL, "nomad_train_batch: contexts must be in strat mode");
-        if (c->p.device != cs[0]->p.device || c->p.storage != cs[0]->p.storage)
-            return fail(NOMAD_EINVAL, "nomad_train_batch: contexts must share device and storage type");
+        if (c->p.device != cs[0]->p.device || c->p.storage != cs[0]->p.storage ||
+            (c->ldf > 128) != (cs[0]->ldf > 128))
+            return fail(NOMAD_EINVAL,
+                        "nomad_train_batch: contexts must share device, storage type and row layout (k <= 128 or > 128)");
         if (c->tr.nnz < 0) return fail(NOMAD_ESTATE, "nomad_train_batch: a context has no training data");
         for (int32_t j = 0; j < i; ++j)
             if (cs[j] == c) return fail(NOMAD_EINVAL, "nomad_train_batch: a context appears twice");
@@ -1171,7 +1182,7 @@ int train_batch(nomad_ctx* const* cs, int32_t count, int64_t epochs, int32_t wav
 
     int cus = 0, per_cu = 0;
     NOMAD_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c0->p.device));
-    NOMAD_HIP_TRY(dispatch_storage(c0->p.storage, [&](auto tag) {
+    NOMAD_HIP_TRY(dispatch_storage(kernel_code(c0->p.storage, c0->ldf), [&](auto tag) {
         return batch_occupancy<decltype(tag)::value>(&per_cu);
     }));
     // default grid: one wave per tile of the widest round, up to the waves
@@ -1238,7 +1249,7 @@ int train_batch(nomad_ctx* const* cs, int32_t count, int64_t epochs, int32_t wav
     ba.nseg = int32_t(nseg);
     ba.deadline = uint64_t(watchdog_seconds() * c0->wall_hz);
     NOMAD_HIP_TRY(hipEventRecord(c0->ev0, c0->stream));
-    hipError_t e = dispatch_storage(c0->p.storage, [&](auto tag) {
+    hipError_t e = dispatch_storage(kernel_code(c0->p.storage, c0->ldf), [&](auto tag) {
         return launch_batch<decltype(tag)::value>(ba, blocks, c0->stream);
     });
     if (e != hipSuccess) return fail(NOMAD_EHIP, std::string("batch launch: ") + hipGetErrorString(e));
@@ -1297,7 +1308,7 @@ int eval_one(nomad_ctx* c, const DevCsc& d, double* sse, bool fixed) {
     a.nnz = d.nnz;
     a.chunk = chunk;
     a.partial = partial;
-    hipError_t e = dispatch_storage(c->p.storage,
+    hipError_t e = dispatch_storage(kernel_code(c->p.storage, c->ldf),
                                     [&](auto t) { return launch_sse<decltype(t)::value>(a, waves, c->stream); });
     if (e != hipSuccess) return fail(NOMAD_EHIP, std::string("sse launch: ") + hipGetErrorString(e));
     std::vector<double> part(static_cast<size_t>(waves));
@@ -1321,7 +1332,8 @@ int nomad_create(const nomad_params* p, nomad_ctx** out) {
     if (!p || !out) return fail(NOMAD_EINVAL, "nomad_create: null argument");
     *out = nullptr;
     if (p->m <= 0 || p->n <= 0 || p->k <= 0) return fail(NOMAD_EINVAL, "nomad_create: m, n, k must be positive");
-    if (p->k > 128) return fail(NOMAD_EINVAL, "nomad_create: k > 128 is not supported (one element pair per lane)");
+    if (p->k > 256)
+        return fail(NOMAD_EINVAL, "nomad_create: k > 256 is not supported (at most two element pairs per lane)");
     if (p->storage < NOMAD_STORE_FP32 || p->storage > NOMAD_STORE_FP64)
         return fail(NOMAD_EINVAL, "nomad_create: unknown storage type");
     if (p->mode != NOMAD_MODE_ASYNC && p->mode != NOMAD_MODE_STRAT)
@@ -1374,7 +1386,7 @@ int nomad_create(const nomad_params* p, nomad_ctx** out) {
                 nw = choice;
                 break;
             }
-        hipError_t oe = dispatch_storage(p->storage, [&](auto t) {
+        hipError_t oe = dispatch_storage(kernel_code(p->storage, (p->k + 1) & ~1), [&](auto t) {
             return async_occupancy<decltype(t)::value>(nw, p->rounding, &per_cu);
         });
         if (oe != hipSuccess || per_cu < 1) return bail(fail(NOMAD_EHIP, "nomad_create: occupancy query failed"));

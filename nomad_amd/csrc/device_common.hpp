@@ -205,6 +205,32 @@ struct Store<NOMAD_STORE_FP32> {
     // rows of a block through one buffer resource: byte offset voff (per
     // lane, out of range for lanes past the row) + soff (row within block)
     static constexpr int kPairBytes = 8;
+
+    // ---- layout helpers shared with the wide layout (see Wide below) ----
+    static constexpr int kNP = 1;  // element pairs per lane
+    using voff_t = int;            // per-lane byte offset of a row's pair(s) in a buffer
+    static __device__ __forceinline__ voff_t lane_voff(int lane, int npairs) {
+        return int(lane < npairs ? uint32_t(lane * kPairBytes) : 0x80000000u);
+    }
+    // every lane out of range (a store the range check drops); d selects
+    // one of 64 distinct such offsets
+    static __device__ __forceinline__ voff_t drop_voff(voff_t v, int d) {
+        return int(uint32_t(v) + 0x7FFFFFFFu) - d * 1024;
+    }
+    // the lane's pair(s) at a given byte offset
+    static __device__ __forceinline__ voff_t voff_bytes(int off) { return off; }
+    static __device__ __forceinline__ pair zero() { return pair{acc(0), acc(0)}; }
+    static __device__ __forceinline__ pair select(bool c, pair a, pair b) {
+        return pair{c ? a.x : b.x, c ? a.y : b.y};
+    }
+    // a column factor held in LDS as acc[2 * 64 * kNP]
+    static __device__ __forceinline__ pair lds_get(const acc* base, int lane) {
+        return pair{base[2 * lane], base[2 * lane + 1]};
+    }
+    static __device__ __forceinline__ void lds_put(acc* base, int lane, pair p) {
+        base[2 * lane] = p.x;
+        base[2 * lane + 1] = p.y;
+    }
     static __device__ __forceinline__ pair load_b(__amdgpu_buffer_rsrc_t r, int voff, int soff = 0) {
         const u32x2_t v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
         return pair{__uint_as_float(v.x), __uint_as_float(v.y)};
@@ -266,6 +292,32 @@ struct Store<NOMAD_STORE_FP64> {
         __builtin_amdgcn_raw_buffer_store_b128(v, row_rsrc(row, npairs * 16), lane * 16, 0, CP);
     }
     static constexpr int kPairBytes = 16;
+
+    // ---- layout helpers shared with the wide layout (see Wide below) ----
+    static constexpr int kNP = 1;  // element pairs per lane
+    using voff_t = int;            // per-lane byte offset of a row's pair(s) in a buffer
+    static __device__ __forceinline__ voff_t lane_voff(int lane, int npairs) {
+        return int(lane < npairs ? uint32_t(lane * kPairBytes) : 0x80000000u);
+    }
+    // every lane out of range (a store the range check drops); d selects
+    // one of 64 distinct such offsets
+    static __device__ __forceinline__ voff_t drop_voff(voff_t v, int d) {
+        return int(uint32_t(v) + 0x7FFFFFFFu) - d * 1024;
+    }
+    // the lane's pair(s) at a given byte offset
+    static __device__ __forceinline__ voff_t voff_bytes(int off) { return off; }
+    static __device__ __forceinline__ pair zero() { return pair{acc(0), acc(0)}; }
+    static __device__ __forceinline__ pair select(bool c, pair a, pair b) {
+        return pair{c ? a.x : b.x, c ? a.y : b.y};
+    }
+    // a column factor held in LDS as acc[2 * 64 * kNP]
+    static __device__ __forceinline__ pair lds_get(const acc* base, int lane) {
+        return pair{base[2 * lane], base[2 * lane + 1]};
+    }
+    static __device__ __forceinline__ void lds_put(acc* base, int lane, pair p) {
+        base[2 * lane] = p.x;
+        base[2 * lane + 1] = p.y;
+    }
     static __device__ __forceinline__ pair load_b(__amdgpu_buffer_rsrc_t r, int voff, int soff = 0) {
         const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
         return pair{__builtin_bit_cast(double, (uint64_t(v.y) << 32) | v.x),
@@ -332,6 +384,32 @@ struct Store<NOMAD_STORE_BF16> {
         __builtin_amdgcn_raw_buffer_store_b32(word, row_rsrc(row, npairs * 4), lane * 4, 0, CP);
     }
     static constexpr int kPairBytes = 4;
+
+    // ---- layout helpers shared with the wide layout (see Wide below) ----
+    static constexpr int kNP = 1;  // element pairs per lane
+    using voff_t = int;            // per-lane byte offset of a row's pair(s) in a buffer
+    static __device__ __forceinline__ voff_t lane_voff(int lane, int npairs) {
+        return int(lane < npairs ? uint32_t(lane * kPairBytes) : 0x80000000u);
+    }
+    // every lane out of range (a store the range check drops); d selects
+    // one of 64 distinct such offsets
+    static __device__ __forceinline__ voff_t drop_voff(voff_t v, int d) {
+        return int(uint32_t(v) + 0x7FFFFFFFu) - d * 1024;
+    }
+    // the lane's pair(s) at a given byte offset
+    static __device__ __forceinline__ voff_t voff_bytes(int off) { return off; }
+    static __device__ __forceinline__ pair zero() { return pair{acc(0), acc(0)}; }
+    static __device__ __forceinline__ pair select(bool c, pair a, pair b) {
+        return pair{c ? a.x : b.x, c ? a.y : b.y};
+    }
+    // a column factor held in LDS as acc[2 * 64 * kNP]
+    static __device__ __forceinline__ pair lds_get(const acc* base, int lane) {
+        return pair{base[2 * lane], base[2 * lane + 1]};
+    }
+    static __device__ __forceinline__ void lds_put(acc* base, int lane, pair p) {
+        base[2 * lane] = p.x;
+        base[2 * lane + 1] = p.y;
+    }
     static __device__ __forceinline__ pair load_b(__amdgpu_buffer_rsrc_t r, int voff, int soff = 0) {
         const uint32_t u = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
         return pair{bf16_lo(u), bf16_hi(u)};
@@ -380,6 +458,105 @@ struct Store<NOMAD_STORE_BF16> {
         if (lane < npairs) st_dc(reinterpret_cast<uint32_t*>(row) + lane, word);
     }
 };
+
+// ---- rows of more than 128 elements: two element pairs per lane ----
+// Lane l holds element pairs l and l + 64 of the row (elements 2l, 2l + 1
+// and 128 + 2l, 128 + 2l + 1), each stored as in the one-pair layout, so a
+// row is two coalesced accesses per wave; ldf <= 256.  The second pair is
+// the first pair's layout applied to the row from element 128 on.  A lane's
+// partial of a dot product adds the second pair's two products after the
+// first pair's.  Stochastic rounding draws the second pair's bits as
+// mix32(bits of the first pair ^ kWideSalt).
+constexpr int kWideBit = 16;  // storage code | kWideBit selects this layout
+constexpr uint32_t kWideSalt = 0x5BD1E995u;
+__device__ __forceinline__ uint32_t wide_bits(uint32_t rnd) { return mix32(rnd ^ kWideSalt); }
+
+template <class BP>
+struct WPair {
+    BP a, b;
+};
+struct WVoff {
+    int a, b;
+};
+
+template <class B>
+struct Wide {
+    using elem = typename B::elem;
+    using acc = typename B::acc;
+    using rval = typename B::rval;
+    using pair = WPair<typename B::pair>;
+    using raw = WPair<typename B::raw>;
+    using voff_t = WVoff;
+    static constexpr int kNP = 2;
+    static constexpr int kPairBytes = B::kPairBytes;        // per element pair
+    static constexpr int kHalfBytes = kWave * B::kPairBytes;  // pair l to pair l + 64
+    static constexpr int kHalfElems = 2 * kWave;
+
+    template <int CP = 0>
+    static __device__ __forceinline__ pair load(const elem* row, int lane, int npairs) {
+        return pair{B::template load<CP>(row, lane, npairs),
+                    B::template load<CP>(row + kHalfElems, lane, npairs - kWave)};
+    }
+    template <int CP = 0>
+    static __device__ __forceinline__ void store(elem* row, int lane, int npairs, pair p, uint32_t rnd, int rm) {
+        B::template store<CP>(row, lane, npairs, p.a, rnd, rm);
+        B::template store<CP>(row + kHalfElems, lane, npairs - kWave, p.b, wide_bits(rnd), rm);
+    }
+    static __device__ __forceinline__ voff_t lane_voff(int lane, int npairs) {
+        return voff_t{B::lane_voff(lane, npairs), int(lane + kWave < npairs ? uint32_t((lane + kWave) * kPairBytes)
+                                                                             : 0x80000000u)};
+    }
+    static __device__ __forceinline__ voff_t drop_voff(voff_t v, int d) {
+        return voff_t{B::drop_voff(v.a, d), B::drop_voff(v.b, d)};
+    }
+    static __device__ __forceinline__ voff_t voff_bytes(int off) { return voff_t{off, off + kHalfBytes}; }
+    template <int CP = 0>
+    static __device__ __forceinline__ raw load_raw_b(__amdgpu_buffer_rsrc_t r, voff_t v, int soff = 0) {
+        return raw{B::template load_raw_b<CP>(r, v.a, soff), B::template load_raw_b<CP>(r, v.b, soff)};
+    }
+    static __device__ __forceinline__ pair unpack(raw u) { return pair{B::unpack(u.a), B::unpack(u.b)}; }
+    static __device__ __forceinline__ raw pack(pair p, uint32_t rnd, int rm) {
+        return raw{B::pack(p.a, rnd, rm), B::pack(p.b, wide_bits(rnd), rm)};
+    }
+    template <int CP = 0>
+    static __device__ __forceinline__ void store_raw_b(__amdgpu_buffer_rsrc_t r, voff_t v, raw p, int soff = 0) {
+        B::template store_raw_b<CP>(r, v.a, p.a, soff);
+        B::template store_raw_b<CP>(r, v.b, p.b, soff);
+    }
+    static __device__ __forceinline__ void store_b(__amdgpu_buffer_rsrc_t r, voff_t v, int soff, pair p,
+                                                   uint32_t rnd, int rm) {
+        B::store_b(r, v.a, soff, p.a, rnd, rm);
+        B::store_b(r, v.b, soff, p.b, wide_bits(rnd), rm);
+    }
+    static __device__ __forceinline__ pair load_dc(const elem* row, int lane, int npairs) {
+        return pair{B::load_dc(row, lane, npairs), B::load_dc(row + kHalfElems, lane, npairs - kWave)};
+    }
+    static __device__ __forceinline__ void store_dc(elem* row, int lane, int npairs, pair p, uint32_t rnd,
+                                                    int rm) {
+        B::store_dc(row, lane, npairs, p.a, rnd, rm);
+        B::store_dc(row + kHalfElems, lane, npairs - kWave, p.b, wide_bits(rnd), rm);
+    }
+    static __device__ __forceinline__ pair zero() { return pair{B::zero(), B::zero()}; }
+    static __device__ __forceinline__ pair select(bool c, pair x, pair y) {
+        return pair{B::select(c, x.a, y.a), B::select(c, x.b, y.b)};
+    }
+    static __device__ __forceinline__ pair lds_get(const acc* base, int lane) {
+        return pair{B::lds_get(base, lane), B::lds_get(base + kHalfElems, lane)};
+    }
+    static __device__ __forceinline__ void lds_put(acc* base, int lane, pair p) {
+        B::lds_put(base, lane, p.a);
+        B::lds_put(base + kHalfElems, lane, p.b);
+    }
+};
+template <>
+struct Store<NOMAD_STORE_FP32 | kWideBit> : Wide<Store<NOMAD_STORE_FP32>> {};
+template <>
+struct Store<NOMAD_STORE_BF16 | kWideBit> : Wide<Store<NOMAD_STORE_BF16>> {};
+template <>
+struct Store<NOMAD_STORE_FP64 | kWideBit> : Wide<Store<NOMAD_STORE_FP64>> {};
+
+// the storage type of a kernel's storage code (without the layout bit)
+constexpr int base_storage(int s) { return s & ~kWideBit; }
 
 // Fused multiply-add in the arithmetic type itself (a single rounding in
 // FP32 for float, in FP64 for double).
@@ -433,6 +610,60 @@ template <class A>
 __device__ __forceinline__ A wave_dot(A hx, A hy, A wx, A wy) {
     A p = hx * wx;
     p = fma_t(hy, wy, p);
+    return wave_tree_sum(p);
+}
+
+// The same three functions for two pairs per lane: the partial takes the
+// second pair's products after the first's; the updates are per element.
+template <class BP, class A>
+__device__ __forceinline__ void sgd_step_b(WPair<BP>& w, WPair<BP>& h, A r, A sel63, A c1, A nstep) {
+    A p = fma_t(h.a.x, w.a.x, -r * sel63);
+    p = fma_t(h.a.y, w.a.y, p);
+    p = fma_t(h.b.x, w.b.x, p);
+    p = fma_t(h.b.y, w.b.y, p);
+    const A c2 = nstep * wave_tree_sum(p);
+    const WPair<BP> w0 = w;
+    w.a.x = fma_t(c2, h.a.x, c1 * w.a.x);
+    w.a.y = fma_t(c2, h.a.y, c1 * w.a.y);
+    w.b.x = fma_t(c2, h.b.x, c1 * w.b.x);
+    w.b.y = fma_t(c2, h.b.y, c1 * w.b.y);
+    h.a.x = fma_t(c2, w0.a.x, c1 * h.a.x);
+    h.a.y = fma_t(c2, w0.a.y, c1 * h.a.y);
+    h.b.x = fma_t(c2, w0.b.x, c1 * h.b.x);
+    h.b.y = fma_t(c2, w0.b.y, c1 * h.b.y);
+}
+
+template <class BP, class A>
+__device__ __forceinline__ A sgd_step(WPair<BP>& w, WPair<BP>& h, A r, A step, A reg) {
+    A p = h.a.x * w.a.x;
+    p = fma_t(h.a.y, w.a.y, p);
+    p = fma_t(h.b.x, w.b.x, p);
+    p = fma_t(h.b.y, w.b.y, p);
+    const A err = wave_tree_sum(p) - r;
+    auto upd = [&](A& wv, A& hv) {
+        const A aa = fma_t(err, hv, reg * wv);
+        const A bb = fma_t(err, wv, reg * hv);
+        wv = fma_t(-step, aa, wv);
+        hv = fma_t(-step, bb, hv);
+    };
+    upd(w.a.x, h.a.x);
+    upd(w.a.y, h.a.y);
+    upd(w.b.x, h.b.x);
+    upd(w.b.y, h.b.y);
+    return err;
+}
+
+// dot product of a lane's pair(s) over the wave
+template <class P>
+__device__ __forceinline__ auto wave_dot_p(const P& h, const P& w) {
+    return wave_dot(h.x, h.y, w.x, w.y);
+}
+template <class BP>
+__device__ __forceinline__ auto wave_dot_p(const WPair<BP>& h, const WPair<BP>& w) {
+    auto p = h.a.x * w.a.x;
+    p = fma_t(h.a.y, w.a.y, p);
+    p = fma_t(h.b.x, w.b.x, p);
+    p = fma_t(h.b.y, w.b.y, p);
     return wave_tree_sum(p);
 }
 

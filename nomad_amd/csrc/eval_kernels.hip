@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void sse_kernel(SseArgs a) {
                             h = T::load(H + int64_t(j) * a.ldf, lane, npairs);
                         }
                         const Aa r = Aa(readlane_val(myval, t));
-                        const Aa err = wave_dot<Aa>(h.x, h.y, w[d].x, w[d].y) - r;
+                        const Aa err = wave_dot_p(h, w[d]) - r;
                         sse += double(err) * double(err);
                     }
                     const int32_t tn = min(t + kEvalPrefetch, cnt - 1);
@@ -88,3 +88,6 @@ __global__ __launch_bounds__(256) void sse_kernel(SseArgs a) {
 template __global__ void sse_kernel<NOMAD_STORE_FP32>(SseArgs);
 template __global__ void sse_kernel<NOMAD_STORE_BF16>(SseArgs);
 template __global__ void sse_kernel<NOMAD_STORE_FP64>(SseArgs);
+template __global__ void sse_kernel<NOMAD_STORE_FP32 | kWideBit>(SseArgs);
+template __global__ void sse_kernel<NOMAD_STORE_BF16 | kWideBit>(SseArgs);
+template __global__ void sse_kernel<NOMAD_STORE_FP64 | kWideBit>(SseArgs);

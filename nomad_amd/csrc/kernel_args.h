@@ -51,7 +51,9 @@ constexpr int kQStride = 32;
 // waves bf16 ran 19 % and fp32 17 % faster with 128 slots than with 64.
 template <int S, int NW>
 constexpr int async_slots() {
-    return S == 2 ? 32 : NW == 8 ? 128 : 64;
+    // (S & 16: rows of more than 128 elements, whose factors take twice the
+    // room per slot)
+    return (S & 16) ? ((S & 15) == 2 ? 32 : NW == 8 ? 64 : 32) : (S == 2 ? 32 : NW == 8 ? 128 : 64);
 }
 
 // per-workgroup counters of the async kernel

@@ -136,7 +136,7 @@ bool parse_args(int argc, char** argv, Options& o) {
         {"tm_error", 'e', "accepted, no effect", Spec::kIgnoredValue, &ignored_d},
         {"tm_interval", 0, "accepted, no effect", Spec::kIgnoredValue, &ignored_d},
         {"tm_portion", 0, "accepted, no effect", Spec::kIgnoredValue, &ignored_d},
-        {"dim", 0, "dimension of latent space (100, at most 128)", Spec::kInt, &o.dim},
+        {"dim", 0, "dimension of latent space (100, at most 256)", Spec::kInt, &o.dim},
         {"reuse", 0, "accepted, no effect", Spec::kIgnoredValue, &ignored_d},
         {"pause", 0, "pause 3 s after each timeout (true)", Spec::kBool, &o.pause},
         {"random_send", 0, "accepted, no effect at one rank", Spec::kIgnoredValue, &ignored_d},
@@ -271,7 +271,7 @@ bool parse_args(int argc, char** argv, Options& o) {
                 bad = true;
                 break;
             }
-        if (o.dim < 1 || o.dim > 128 || o.nthreads < 0 || o.ptoken < 0) bad = true;
+        if (o.dim < 1 || o.dim > 256 || o.nthreads < 0 || o.ptoken < 0) bad = true;
         if (o.cp_type != "none" && o.cp_type != "sync" && o.cp_type != "async" && o.cp_type != "vs") {
             std::cerr << "checkpoint type must be none, sync, async or vs." << std::endl;
             bad = true;

@@ -42,27 +42,47 @@ struct ObjTask {
     int32_t line;
 };
 
-template <class A>
-struct Acc;
-template <>
-struct Acc<float> {
-    using pair = Store<NOMAD_STORE_FP32>::pair;
-    using T = Store<NOMAD_STORE_FP32>;
+// the storage traits of the arithmetic type; WIDE: rows of more than 128
+// elements, element pairs l and l + 64 in lane l
+template <class A, bool WIDE>
+struct Acc {
+    using T = Store<(sizeof(A) == 8 ? NOMAD_STORE_FP64 : NOMAD_STORE_FP32) | (WIDE ? kWideBit : 0)>;
+    using pair = typename T::pair;
 };
-template <>
-struct Acc<double> {
-    using pair = Store<NOMAD_STORE_FP64>::pair;
-    using T = Store<NOMAD_STORE_FP64>;
-};
+
+// acc += e * y per element, and the lane's elements written to a plain row
+// (lanes past the row write nothing)
+template <class P, class A>
+__device__ __forceinline__ void axpy_p(A e, const P& y, P& acc) {
+    acc.x = fma_t(e, y.x, acc.x);
+    acc.y = fma_t(e, y.y, acc.y);
+}
+template <class BP, class A>
+__device__ __forceinline__ void axpy_p(A e, const WPair<BP>& y, WPair<BP>& acc) {
+    axpy_p(e, y.a, acc.a);
+    axpy_p(e, y.b, acc.b);
+}
+template <class P, class A>
+__device__ __forceinline__ void row_put(A* dst, int lane, int npairs, const P& v) {
+    if (lane < npairs) {
+        dst[2 * lane] = v.x;
+        dst[2 * lane + 1] = v.y;
+    }
+}
+template <class BP, class A>
+__device__ __forceinline__ void row_put(A* dst, int lane, int npairs, const WPair<BP>& v) {
+    row_put(dst, lane, npairs, v.a);
+    row_put(dst + 2 * kWave, lane, npairs - kWave, v.b);
+}
 
 // One wave per task: own = the task line's factor (column j: h_j, row i: w_i),
 // other = the opposite side.  part[t] (ldf values) = sum over the task's
 // ratings of e * other_row; sq[t] = sum of e^2 (FP64).
-template <class A>
+template <class A, bool WIDE>
 __global__ void obj_pass(const ObjTask* tasks, int ntasks, const int32_t* idx, const A* val, const A* own,
                          const A* other, int ldf, A* part, double* sq) {
-    using T = typename Acc<A>::T;
-    using P = typename Acc<A>::pair;
+    using T = typename Acc<A, WIDE>::T;
+    using P = typename Acc<A, WIDE>::pair;
     const int lane = threadIdx.x & 63;
     // wave-uniform (scalar) task index: the task's line then addresses its
     // row through scalar registers
@@ -72,7 +92,7 @@ __global__ void obj_pass(const ObjTask* tasks, int ntasks, const int32_t* idx, c
     for (int t = wave; t < ntasks; t += nwaves) {
         const ObjTask tk = tasks[t];
         const P x = T::load(own + int64_t(tk.line) * ldf, lane, npairs);
-        P acc{A(0), A(0)};
+        P acc = T::zero();
         double s2 = 0;
         for (int base = 0; base < tk.len; base += kWave) {
             const int cnt = min(kWave, tk.len - base);
@@ -88,9 +108,8 @@ __global__ void obj_pass(const ObjTask* tasks, int ntasks, const int32_t* idx, c
                 for (int d = 0; d < kPre; ++d) {
                     const int q = q0 + d;
                     if (q < cnt) {
-                        const A e = wave_dot(x.x, x.y, y[d].x, y[d].y) - A(readlane_val(myval, q));
-                        acc.x = fma_t(e, y[d].x, acc.x);
-                        acc.y = fma_t(e, y[d].y, acc.y);
+                        const A e = wave_dot_p(x, y[d]) - A(readlane_val(myval, q));
+                        axpy_p(e, y[d], acc);
                         s2 += double(e) * double(e);
                     }
                     const int qn = min(q + kPre, cnt - 1);
@@ -98,11 +117,7 @@ __global__ void obj_pass(const ObjTask* tasks, int ntasks, const int32_t* idx, c
                 }
             }
         }
-        A* dst = part + int64_t(t) * ldf;
-        if (lane < npairs) {
-            dst[2 * lane] = acc.x;
-            dst[2 * lane + 1] = acc.y;
-        }
+        row_put(part + int64_t(t) * ldf, lane, npairs, acc);
         if (lane == 0) sq[t] = s2;
     }
 }
@@ -211,7 +226,7 @@ extern "C" int nomad_obj_create(const nomad_obj_params* p, const int64_t* colptr
     if (!p || !colptr || !rowidx || !val || !out) return fail(NOMAD_EINVAL, "nomad_obj_create: null argument");
     *out = nullptr;
     const int m = p->m, n = p->n, k = p->k;
-    if (m <= 0 || n <= 0 || k <= 0 || k > 128) return fail(NOMAD_EINVAL, "nomad_obj_create: need m, n > 0, 0 < k <= 128");
+    if (m <= 0 || n <= 0 || k <= 0 || k > 256) return fail(NOMAD_EINVAL, "nomad_obj_create: need m, n > 0, 0 < k <= 256");
     if (p->storage != NOMAD_STORE_FP32 && p->storage != NOMAD_STORE_FP64)
         return fail(NOMAD_EINVAL, "nomad_obj_create: storage must be FP32 or FP64");
     if (nnz < 0 || colptr[0] != 0 || colptr[n] != nnz) return fail(NOMAD_EINVAL, "nomad_obj_create: bad colptr");
@@ -291,8 +306,10 @@ int obj_eval_t(nomad_obj_ctx* c, const A* W, const A* H, double reg, A* gW, A* g
         return unsigned(std::max<int64_t>(1, std::min<int64_t>(16384, (count + threads - 1) / threads)));
     };
     // columns: dL/dH, squared errors, column norms
-    hipLaunchKernelGGL((obj_pass<A>), dim3(grid(int64_t(c->nct) * 64, 256)), dim3(256), 0, s, c->ctasks, c->nct,
-                       c->rowidx, static_cast<const A*>(c->cval), H, W, ldf, part, c->sq);
+    // (rows of more than 128 elements: two element pairs per lane)
+    auto pass = ldf > 128 ? obj_pass<A, true> : obj_pass<A, false>;
+    hipLaunchKernelGGL(pass, dim3(grid(int64_t(c->nct) * 64, 256)), dim3(256), 0, s, c->ctasks, c->nct, c->rowidx,
+                       static_cast<const A*>(c->cval), H, W, ldf, part, c->sq);
     hipLaunchKernelGGL(obj_sum_stage1, dim3(kRedBlocks), dim3(256), 0, s, int64_t(c->nct), c->sq, c->red);
     hipLaunchKernelGGL(obj_sum_stage2, dim3(1), dim3(256), 0, s, kRedBlocks, c->red, c->out, 0);
     hipLaunchKernelGGL((obj_finalize<A>), dim3(grid(int64_t(n) * ldf, 256)), dim3(256), 0, s, n, ldf, c->ctask_ptr,
@@ -300,8 +317,8 @@ int obj_eval_t(nomad_obj_ctx* c, const A* W, const A* H, double reg, A* gW, A* g
     hipLaunchKernelGGL(obj_sum_stage1, dim3(kRedBlocks), dim3(256), 0, s, int64_t(n) * ldf, c->nrm, c->red);
     hipLaunchKernelGGL(obj_sum_stage2, dim3(1), dim3(256), 0, s, kRedBlocks, c->red, c->out, 1);
     // rows: dL/dW, row norms
-    hipLaunchKernelGGL((obj_pass<A>), dim3(grid(int64_t(c->nrt) * 64, 256)), dim3(256), 0, s, c->rtasks, c->nrt,
-                       c->colidx, static_cast<const A*>(c->rval), W, H, ldf, part, c->sq);
+    hipLaunchKernelGGL(pass, dim3(grid(int64_t(c->nrt) * 64, 256)), dim3(256), 0, s, c->rtasks, c->nrt, c->colidx,
+                       static_cast<const A*>(c->rval), W, H, ldf, part, c->sq);
     hipLaunchKernelGGL((obj_finalize<A>), dim3(grid(int64_t(m) * ldf, 256)), dim3(256), 0, s, m, ldf, c->rtask_ptr,
                        c->rowptr, part, W, A(reg), gW, c->nrm);
     hipLaunchKernelGGL(obj_sum_stage1, dim3(kRedBlocks), dim3(256), 0, s, int64_t(m) * ldf, c->nrm, c->red);

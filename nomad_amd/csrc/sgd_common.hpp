@@ -60,27 +60,30 @@ __device__ __forceinline__ void row_store(typename Store<S>::elem* row, int lane
 // W rows of one row block [rlo, rlo + rows) through one buffer resource:
 // lane l's pair of row r sits at byte offset wvoff + (r - rlo) * rowbytes,
 // wvoff = l * pair bytes (or 2^31 for lanes past the row, so that they are
-// out of range), and kDrop puts every lane out of range (a store the range
-// check drops).  The row part (r - rlo) * rowbytes goes to the scalar offset
-// of a load or store (it is always a row of the block; the hardware range
-// check covers the per-lane offset only).
+// out of range; two such offsets in the two-pair layout), and drop() puts
+// every lane out of range (a store the range check drops).  The row part
+// (r - rlo) * rowbytes goes to the scalar offset of a load or store (it is
+// always a row of the block; the hardware range check covers the per-lane
+// offset only).
+template <int S>
 struct RowBlock {
+    using voff_t = typename Store<S>::voff_t;
     __amdgpu_buffer_rsrc_t rsrc;
-    uint32_t wvoff;
+    voff_t wvoff;
     int32_t rlo;
     int32_t rowbytes;
     __device__ __forceinline__ int soff(int32_t r) const { return (r - rlo) * rowbytes; }
-    __device__ __forceinline__ int drop(int d = 0) const { return int(wvoff + 0x7FFFFFFFu) - d * 1024; }
+    __device__ __forceinline__ voff_t drop(int d = 0) const { return Store<S>::drop_voff(wvoff, d); }
 };
 
 template <int S>
-__device__ __forceinline__ RowBlock row_block(const SgdArgs& a, int32_t rlo, int32_t rhi, int lane) {
+__device__ __forceinline__ RowBlock<S> row_block(const SgdArgs& a, int32_t rlo, int32_t rhi, int lane) {
     using T = Store<S>;
-    RowBlock rb;
+    RowBlock<S> rb;
     rb.rowbytes = a.ldf * int(sizeof(typename T::elem));
     rb.rsrc = __builtin_amdgcn_make_buffer_rsrc(static_cast<typename T::elem*>(a.W) + int64_t(rlo) * a.ldf, 0,
                                                 (rhi - rlo) * rb.rowbytes, 0x00020000);
-    rb.wvoff = lane < (a.ldf >> 1) ? uint32_t(lane * T::kPairBytes) : 0x80000000u;
+    rb.wvoff = T::lane_voff(lane, a.ldf >> 1);
     rb.rlo = rlo;
     return rb;
 }
@@ -90,7 +93,7 @@ __device__ __forceinline__ RowBlock row_block(const SgdArgs& a, int32_t rlo, int
 // (past a visit's end the prefetch slots re-read its last row)
 template <int S, bool DC>
 constexpr int strat_depth() {
-    return (S == NOMAD_STORE_FP64 || DC) ? 8 : 16;
+    return (base_storage(S) == NOMAD_STORE_FP64 || DC) ? 8 / Store<S>::kNP : 16 / Store<S>::kNP;
 }
 
 // SGD steps for the first `cnt` (<= 64) ratings held one per lane in
@@ -102,7 +105,7 @@ constexpr int strat_depth() {
 // waits are exactly the prefetch distance.  A column's rows are distinct, so
 // no prefetched row is updated before its rating uses it.
 template <int S, bool DC, int RM>
-__device__ __forceinline__ void run_chunk(const SgdArgs& a, const RowBlock& rb, typename Store<S>::pair& h,
+__device__ __forceinline__ void run_chunk(const SgdArgs& a, const RowBlock<S>& rb, typename Store<S>::pair& h,
                                           int32_t myrow, typename Store<S>::rval myval, int32_t cnt,
                                           typename Store<S>::acc step, typename Store<S>::acc reg, uint32_t sr_b,
                                           uint32_t lkey) {
@@ -114,8 +117,8 @@ __device__ __forceinline__ void run_chunk(const SgdArgs& a, const RowBlock& rb, 
     typename T::raw wp[kD];
 #pragma unroll
     for (int d = 0; d < kD; ++d) {
-        T::template store_raw_b<CP>(rb.rsrc, rb.drop(d), T::pack(P{Aa(0), Aa(0)}, 0u, 0));
-        wp[d] = T::template load_raw_b<CP>(rb.rsrc, int(rb.wvoff), rb.soff(readlane_i32(myrow, min(d, cnt - 1))));
+        T::template store_raw_b<CP>(rb.rsrc, rb.drop(d), T::pack(T::zero(), 0u, 0));
+        wp[d] = T::template load_raw_b<CP>(rb.rsrc, rb.wvoff, rb.soff(readlane_i32(myrow, min(d, cnt - 1))));
     }
     // groups of kG ratings; a group that starts past the chunk leaves both
     // loops at once (an exit edge keeps the loop's wait counts exact)
@@ -133,13 +136,12 @@ __device__ __forceinline__ void run_chunk(const SgdArgs& a, const RowBlock& rb, 
                 const int32_t i = readlane_i32(myrow, uc);
                 P w = T::unpack(wp[d]);
                 const P h0 = h;
-                sgd_step<P, Aa>(w, h, Aa(readlane_val(myval, uc)), step, reg);
-                h.x = real ? h.x : h0.x;
-                h.y = real ? h.y : h0.y;
+                sgd_step(w, h, Aa(readlane_val(myval, uc)), step, reg);
+                h = T::select(real, h, h0);
                 const uint32_t rnd = sr_bits_k(a.sr_seed, uint32_t(i), sr_b, lkey);
-                T::template store_raw_b<CP>(rb.rsrc, real ? int(rb.wvoff) : rb.drop(), T::pack(w, rnd, RM),
+                T::template store_raw_b<CP>(rb.rsrc, real ? rb.wvoff : rb.drop(), T::pack(w, rnd, RM),
                                             real ? rb.soff(i) : 0);
-                wp[d] = T::template load_raw_b<CP>(rb.rsrc, int(rb.wvoff),
+                wp[d] = T::template load_raw_b<CP>(rb.rsrc, rb.wvoff,
                                                    rb.soff(readlane_i32(myrow, min(t + kD, cnt - 1))));
             }
         }
@@ -152,7 +154,7 @@ chunk_done:;
 // visit's length holding the last rating's), so that the caller can fetch
 // them ahead, during the previous visit.
 template <int S, bool DC, int RM>
-__device__ __forceinline__ void run_visit_pre(const SgdArgs& a, const RowBlock& rb, typename Store<S>::pair& h,
+__device__ __forceinline__ void run_visit_pre(const SgdArgs& a, const RowBlock<S>& rb, typename Store<S>::pair& h,
                                               int32_t j, int64_t start, int32_t len, int32_t row0,
                                               typename Store<S>::rval val0, typename Store<S>::acc step,
                                               typename Store<S>::acc reg, uint32_t count, uint32_t lkey) {
@@ -212,7 +214,7 @@ __device__ __forceinline__ void run_chunk_lds(const SgdArgs& a, typename Store<S
             if (t >= cnt) goto chunk_done;
             const int32_t i = readlane_i32(myrow, t);
             P w = T::unpack(wp[d]);
-            sgd_step<P, Aa>(w, h, Aa(readlane_val(myval, t)), step, reg);
+            sgd_step(w, h, Aa(readlane_val(myval, t)), step, reg);
             const uint32_t rnd = sr_bits_k(a.sr_seed, uint32_t(i), sr_b, lkey);
             if (inr) wl[readlane_i32(myslot, t) * npairs + lane] = T::pack(w, rnd, RM);
             wp[d] = lds_row_load<S>(wl, readlane_i32(myslot, min(t + kD, cnt - 1)) * npairs + lane, inr);

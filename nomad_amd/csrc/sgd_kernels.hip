@@ -67,9 +67,11 @@ struct TileLds {
     using T = Store<S>;
     using raw = typename T::raw;
     using R = typename T::rval;
-    static constexpr int kW = kTileLdsBytes / int(sizeof(raw));  // staged W pairs
-    static constexpr int kK = S == NOMAD_STORE_FP64 ? 4 : 8;     // visits per group
-    static constexpr int kQ = S == NOMAD_STORE_BF16 ? 4 : (S == NOMAD_STORE_FP32 ? 2 : 1);  // x64 ratings
+    static constexpr int kB = base_storage(S);
+    // staged W (the two-pair layout does not stage W: its rows take the HBM path)
+    static constexpr int kW = T::kNP == 1 ? kTileLdsBytes / int(sizeof(raw)) : kWave;
+    static constexpr int kK = kB == NOMAD_STORE_FP64 ? 4 : 8;  // visits per group
+    static constexpr int kQ = kB == NOMAD_STORE_BF16 ? 4 : (kB == NOMAD_STORE_FP32 ? 2 : 1);  // x64 ratings
     struct Group {
         raw h[kK * kWave];      // visit u's h pair of lane l at [u * 64 + l]
         int32_t key[kQ * kWave];  // rating's LDS slot (W staged) or row
@@ -104,10 +106,10 @@ __device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep,
     const gptr<R> rv = gbl(static_cast<const R*>(a.val));
     const int npairs = a.ldf >> 1;
     const int32_t rlo = b * a.rpb, rhi = min(a.m, (b + 1) * a.rpb);
-    const RowBlock rb = row_block<S>(a, rlo, rhi, lane);
+    const RowBlock<S> rb = row_block<S>(a, rlo, rhi, lane);
     const uint32_t lkey = sr_lane_key(a.sr_seed, uint32_t(lane));
     const int32_t staged = nrows * npairs;
-    const bool in_lds = nrows <= kStageRows && staged <= TL::kW;
+    const bool in_lds = T::kNP == 1 && nrows <= kStageRows && staged <= TL::kW;
     constexpr int CP = DC ? kCpDevice : 0;
 
     // Group descriptors: lane u <= kK holds visit f + u (clamped to v1, the
@@ -190,7 +192,8 @@ __device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep,
 #pragma unroll
             for (int u = 0; u < kK; ++u) {
                 const E* row = H + int64_t(readlane_i32(d.col, min(u, nv - 1))) * a.ldf;
-                sg.h[u] = T::template load_raw_b<CP>(row_rsrc(row, npairs * T::kPairBytes), lane * T::kPairBytes);
+                sg.h[u] = T::template load_raw_b<CP>(row_rsrc(row, npairs * T::kPairBytes),
+                                                     T::lane_voff(lane, kWave * T::kNP));
             }
             const int64_t gs = start_of(d, 0);
             const int32_t n = int32_t(min(start_of(d, nv) - gs, int64_t(kQ * kWave)));
@@ -220,7 +223,7 @@ __device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep,
                 const int32_t q = u * kWave + lane;
                 int32_t pr;
                 const int32_t r = staged_row(q, pr);
-                t[u] = T::template load_raw_b<CP>(rb.rsrc, q < staged ? rb.soff(r) + pr * T::kPairBytes : rb.drop());
+                t[u] = T::template load_raw_b<CP>(rb.rsrc, q < staged ? T::voff_bytes(rb.soff(r) + pr * T::kPairBytes) : rb.drop());
             }
 #pragma unroll
             for (int u = 0; u < kU; ++u) {
@@ -279,17 +282,22 @@ __device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep,
             d2 = d3;
         }
     };
-    if (in_lds)
-        run(std::true_type{});
-    else
+    if constexpr (T::kNP == 1) {
+        if (in_lds)
+            run(std::true_type{});
+        else
+            run(std::false_type{});
+    } else {
         run(std::false_type{});
-    if (in_lds) {
+    }
+    if constexpr (T::kNP == 1) {
+        if (!in_lds) return;
         // the rows back to W (the range check drops the lanes past them)
         for (int32_t q0 = 0; q0 < staged; q0 += kWave) {
             const int32_t q = q0 + lane;
             int32_t pr;
             const int32_t r = staged_row(q, pr);
-            T::template store_raw_b<CP>(rb.rsrc, q < staged ? rb.soff(r) + pr * T::kPairBytes : rb.drop(),
+            T::template store_raw_b<CP>(rb.rsrc, q < staged ? T::voff_bytes(rb.soff(r) + pr * T::kPairBytes) : rb.drop(),
                                         L.w[min(q, staged - 1)]);
         }
     }
@@ -389,7 +397,7 @@ __global__ __launch_bounds__(256) void sgd_batch_kernel(BatchArgs ba) {
             }
             __atomic_signal_fence(__ATOMIC_SEQ_CST);  // no row access moves above the wait
         }
-        if constexpr (S == NOMAD_STORE_BF16) {  // problems may round differently
+        if constexpr (base_storage(S) == NOMAD_STORE_BF16) {  // problems may round differently
             if (a.round_mode == NOMAD_ROUND_STOCHASTIC)
                 strat_tile<S, true, NOMAD_ROUND_STOCHASTIC>(a, b, r % B, a.epoch + r / B, tl[wave]);
             else
@@ -410,4 +418,12 @@ template __global__ void sgd_strat_kernel<NOMAD_STORE_FP64, NOMAD_ROUND_NEAREST>
 template __global__ void sgd_batch_kernel<NOMAD_STORE_FP32>(BatchArgs);
 template __global__ void sgd_batch_kernel<NOMAD_STORE_BF16>(BatchArgs);
 template __global__ void sgd_batch_kernel<NOMAD_STORE_FP64>(BatchArgs);
+// rows of more than 128 elements (two element pairs per lane)
+template __global__ void sgd_strat_kernel<NOMAD_STORE_FP32 | kWideBit, NOMAD_ROUND_NEAREST>(SgdArgs);
+template __global__ void sgd_strat_kernel<NOMAD_STORE_BF16 | kWideBit, NOMAD_ROUND_NEAREST>(SgdArgs);
+template __global__ void sgd_strat_kernel<NOMAD_STORE_BF16 | kWideBit, NOMAD_ROUND_STOCHASTIC>(SgdArgs);
+template __global__ void sgd_strat_kernel<NOMAD_STORE_FP64 | kWideBit, NOMAD_ROUND_NEAREST>(SgdArgs);
+template __global__ void sgd_batch_kernel<NOMAD_STORE_FP32 | kWideBit>(BatchArgs);
+template __global__ void sgd_batch_kernel<NOMAD_STORE_BF16 | kWideBit>(BatchArgs);
+template __global__ void sgd_batch_kernel<NOMAD_STORE_FP64 | kWideBit>(BatchArgs);
 

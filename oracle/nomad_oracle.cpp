@@ -96,6 +96,9 @@ uint32_t sr_bits(uint32_t seed, uint32_t a, uint32_t b, uint32_t lane) {
 // p = x0*y0 then p = fma(x1, y1, p); the 64 partials are combined as
 //   Q_q = (p[4q]+p[4q+1]) + (p[4q+2]+p[4q+3]);  R_r = (Q[4r+3]+Q[4r+2]) + (Q[4r+1]+Q[4r]);
 //   total = (R3+R2) + (R1+R0).
+// Rows of more than 128 elements (ldf > 128): lane l also holds elements
+// (128+2l, 128+2l+1) (zero past the row) and adds p = fma(x2, y2, p),
+// p = fma(x3, y3, p) before the tree.
 template <class A>
 A gpu_dot(const A* h, const A* w, int ldf) {
     A p[64];
@@ -106,6 +109,12 @@ A gpu_dot(const A* h, const A* w, int ldf) {
             p[l] = std::fma(h[2 * l + 1], w[2 * l + 1], t);
         } else {
             p[l] = A(0);
+        }
+        if (npairs > 64) {
+            const int q = l + 64;
+            const bool in = q < npairs;
+            p[l] = std::fma(in ? h[2 * q] : A(0), in ? w[2 * q] : A(0), p[l]);
+            p[l] = std::fma(in ? h[2 * q + 1] : A(0), in ? w[2 * q + 1] : A(0), p[l]);
         }
     }
     A Q[16], R[4];
@@ -325,7 +334,9 @@ static void strat_replay_bf16(int rounding, uint32_t sr_seed, int m, int n, int 
     std::vector<float> h(ldf), w(ldf);
     auto store = [&](uint16_t* dst, const float* src, uint32_t a, uint32_t bkey) {
         for (int l = 0; l < npairs; ++l) {
-            uint32_t rnd = sr_bits(sr_seed, a, bkey, uint32_t(l));
+            // (pairs 64.. of a wide row: the bits of pair l - 64, rehashed)
+            uint32_t rnd = l < 64 ? sr_bits(sr_seed, a, bkey, uint32_t(l))
+                                  : mix32(sr_bits(sr_seed, a, bkey, uint32_t(l - 64)) ^ 0x5BD1E995u);
             if (rounding == ORC_ROUND_STOCHASTIC) {
                 dst[2 * l] = bf16_sr(src[2 * l], rnd);
                 dst[2 * l + 1] = bf16_sr(src[2 * l + 1], rnd >> 16);

@@ -28,9 +28,10 @@ def dev(a, dtype=torch.float64):
     return torch.tensor(np.asarray(a), dtype=dtype)
 
 
-def test_objective_matches_oracle_fp64(small):
+@pytest.mark.parametrize("k", [15, 201])
+def test_objective_matches_oracle_fp64(small, k):
+    # 15: odd, padded to 16 on the device; 201: two element pairs per lane
     tr, _ = small
-    k = 15  # odd: the library pads rows to 16 on the device
     rng = np.random.default_rng(3)
     W = rng.random((tr.m, k)) / 4
     H = rng.random((tr.n, k)) / 4

@@ -274,7 +274,7 @@ def test_bad_arguments_raise(gpu):
     with pytest.raises(gpu.NomadError):
         gpu.Context(10, 10, 0)
     with pytest.raises(gpu.NomadError):
-        gpu.Context(10, 10, 200)
+        gpu.Context(10, 10, 257)  # k above 256
     ctx = gpu.Context(10, 10, 4, mode="strat", nworkers=2)
     with pytest.raises(gpu.NomadError):
         ctx.train(epochs=1)  # no data uploaded

@@ -225,6 +225,36 @@ def test_gpu_order_close_to_reference_order():
     assert not np.array_equal(Wr, Wg)  # the two orders are really different
 
 
+@pytest.mark.parametrize("k", [200, 256])
+def test_gpu_order_wide_rows(k):
+    """Rows of more than 128 elements: the GPU order takes pairs l and l + 64
+    in lane l.  FP64 agrees with the reference order to rounding (a dropped,
+    doubled or misplaced second pair would not), and FP32 stays within 1e-5
+    of it while really differing from it."""
+    tr, _ = lib.synth(lib.preset(1, 0.15))
+    W0, H0 = lib.init_factors(tr.m, tr.n, k, 4, 12345)
+    Wr, Hr = O.sgd_strat(tr, W0, H0, storage=O.FP64, arith=O.ARITH_REF, B=3, epochs=2, **HYP)
+    Wg, Hg = O.sgd_strat(tr, W0, H0, storage=O.FP64, arith=O.ARITH_GPU, B=3, epochs=2, **HYP)
+    assert np.abs(Wr - Wg).max() <= 1e-12 * np.abs(Wr).max()
+    assert np.abs(Hr - Hg).max() <= 1e-12 * np.abs(Hr).max()
+    Wr, _ = O.sgd_strat(tr, W0, H0, storage=O.FP32, arith=O.ARITH_REF, B=3, epochs=2, **HYP)
+    Wg, _ = O.sgd_strat(tr, W0, H0, storage=O.FP32, arith=O.ARITH_GPU, B=3, epochs=2, **HYP)
+    assert np.abs(Wr - Wg).max() <= 1e-5 * np.abs(Wr).max()
+    assert not np.array_equal(Wr, Wg)
+
+
+def test_sse_gpu_order_wide_rows():
+    """The SSE in the GPU order for k = 200 equals numpy's FP64 dot products
+    to rounding."""
+    tr, te = lib.synth(lib.preset(1, 0.15))
+    k = 200
+    W, H = lib.init_factors(tr.m, tr.n, k, 2, 1)
+    r, c, v = te.to_triplets()
+    want = float(np.sum((np.einsum("ij,ij->i", W[r], H[c]) - v) ** 2))
+    got = O.sse(te, W, H, storage=O.FP64, arith=O.ARITH_GPU)
+    assert abs(got - want) <= 1e-12 * want
+
+
 def test_async_one_thread_is_serial():
     tr, _ = lib.synth(lib.preset(1, 0.1))
     W0, H0 = lib.init_factors(tr.m, tr.n, 8, 1, 4)
