@@ -101,9 +101,11 @@ def test_wide_batch_rejects_mixed_layouts(gpu, tiny):
     b.close()
 
 
-@pytest.mark.parametrize("storage", ["fp32", "bf16", "fp64"])
-def test_wide_async_counts_and_rmse(gpu, small, storage):
-    """k = 200 in async mode (8 waves per workgroup, 64 LDS slots): exactly
+@pytest.mark.parametrize("storage,nworkers", [("fp32", 1024), ("bf16", 1024), ("fp64", 1024), ("bf16", 2048),
+                                              ("fp64", 2048), ("fp32", 4096)])
+def test_wide_async_counts_and_rmse(gpu, small, storage, nworkers):
+    """k = 200 in async mode (1024 workers: 4 waves per workgroup, 32 LDS
+    slots; 2048: 8 waves, 64 slots (32 for fp64); 4096: 16 waves): exactly
     nnz updates per epoch and, after 6 epochs, the test RMSE within 0.5 % of
     the CPU async reference."""
     tr, te = small
@@ -111,8 +113,8 @@ def test_wide_async_counts_and_rmse(gpu, small, storage):
     epochs = 6
     W0, H0 = gpu.init_factors(tr.m, tr.n, k, 4, 12345)
     rnd = "stochastic" if storage == "bf16" else "nearest"
-    print(f"wide async {storage}", flush=True)
-    with _ctx(gpu, tr, te, k, storage=storage, rounding=rnd, mode="async", nworkers=1024) as ctx:
+    print(f"wide async {storage} {nworkers}", flush=True)
+    with _ctx(gpu, tr, te, k, storage=storage, rounding=rnd, mode="async", nworkers=nworkers) as ctx:
         ctx.set_factors(W0, H0)
         for ep in range(epochs):
             st = ctx.train(epochs=1)
