@@ -186,6 +186,10 @@ int nomad_autotune(const nomad_params* params, const int64_t* colptr, const int3
  * trial builds scratch strat contexts of the problems (params as given,
  * nworkers replaced; factors drawn from mt19937_64(params.seed), U(0,
  * 1/sqrt(k))) and times one batched epoch after a warm-up epoch.
+ * A second stage times each problem's own epoch at the common count; the
+ * problems within 30 % of the slowest, slowest first, then try 2x and 4x
+ * that count one at a time, each kept only if it shortens the batched epoch
+ * of all problems by at least 1 %.
  * strata_out[i] receives problem i's chosen count, *updates_per_s the
  * batch's rate with it.  count <= 256. */
 typedef struct nomad_batch_problem {

@@ -172,7 +172,84 @@ extern "C" int nomad_autotune_batch(const nomad_batch_problem* probs, int32_t co
         for (nomad_ctx* c : cs) nomad_destroy(c);
     }
     if (best_B == 0) return fail(last_rc, "nomad_autotune_batch: every trial failed; last error: " + last_msg);
-    for (int32_t i = 0; i < count; ++i) strata_out[i] = std::min(best_B, probs[i].params.m);
+    std::vector<int32_t> Bsel(static_cast<size_t>(count));
+    for (int32_t i = 0; i < count; ++i) Bsel[size_t(i)] = std::min(best_B, probs[i].params.m);
+
+    // Second stage (greedy): a batched launch lasts at least as long as its
+    // slowest problem's chain of sub-epochs, and a sparse problem's chain is
+    // shorter with more strata; but every extra tile costs a ticket and a
+    // publication, so counts are raised one problem at a time.  Each
+    // problem's own epoch (a batched launch of one) is timed at the common
+    // count; the problems within 30 % of the slowest, slowest first, then try
+    // 2x and 4x the common count, each kept only if a batched epoch of all
+    // problems with it is at least 1 % shorter than the best so far (the
+    // shorter of two epochs after a warm-up epoch).
+    auto make = [&](int32_t i, int32_t B, nomad_ctx** out) -> int {
+        nomad_params p = probs[i].params;
+        p.nworkers = B;
+        *out = nullptr;
+        int rc = nomad_create(&p, out);
+        if (rc == NOMAD_OK) rc = nomad_upload_train(*out, probs[i].colptr, probs[i].rowidx, probs[i].val, probs[i].nnz);
+        if (rc == NOMAD_OK) rc = nomad_set_factors(*out, W0[size_t(i)].data(), H0[size_t(i)].data());
+        return rc;
+    };
+    auto timed = [](nomad_ctx** cs, int32_t n, double* sec) -> int {
+        std::vector<nomad_train_stats> st(static_cast<size_t>(n));
+        int rc = nomad_train_batch(cs, n, 1, 0, st.data());
+        double best = 0.0;
+        for (int rep = 0; rep < 2 && rc == NOMAD_OK; ++rep) {
+            rc = nomad_train_batch(cs, n, 1, 0, st.data());
+            if (rc == NOMAD_OK && (rep == 0 || st[0].train_seconds < best)) best = st[0].train_seconds;
+        }
+        *sec = best;
+        return rc;
+    };
+    if (count > 1) {
+        std::vector<nomad_ctx*> cur(static_cast<size_t>(count), nullptr);
+        std::vector<double> own(static_cast<size_t>(count), 0.0);
+        int rc = NOMAD_OK;
+        for (int32_t i = 0; i < count && rc == NOMAD_OK; ++i) {
+            rc = make(i, Bsel[size_t(i)], &cur[size_t(i)]);
+            if (rc == NOMAD_OK) rc = timed(&cur[size_t(i)], 1, &own[size_t(i)]);
+        }
+        double T = 0.0;
+        if (rc == NOMAD_OK) rc = timed(cur.data(), count, &T);
+        if (rc == NOMAD_OK && T > 0.0) {
+            const double own_max = *std::max_element(own.begin(), own.end());
+            std::vector<int32_t> cand;
+            for (int32_t i = 0; i < count; ++i)
+                if (own[size_t(i)] >= 0.7 * own_max) cand.push_back(i);
+            std::stable_sort(cand.begin(), cand.end(),
+                             [&](int32_t x, int32_t y) { return own[size_t(x)] > own[size_t(y)]; });
+            for (int32_t i : cand) {
+                for (int32_t f : {2, 4}) {
+                    const int32_t B = std::min(best_B * f, probs[i].params.m);
+                    if (B <= Bsel[size_t(i)]) break;
+                    nomad_ctx* nc = nullptr;
+                    if (make(i, B, &nc) != NOMAD_OK) {
+                        nomad_destroy(nc);
+                        break;
+                    }
+                    nomad_ctx* old = cur[size_t(i)];
+                    cur[size_t(i)] = nc;
+                    double t = 0.0;
+                    if (timed(cur.data(), count, &t) == NOMAD_OK && t > 0.0 && t < 0.99 * T) {
+                        nomad_destroy(old);
+                        T = t;
+                        Bsel[size_t(i)] = B;
+                    } else {
+                        cur[size_t(i)] = old;
+                        nomad_destroy(nc);
+                    }
+                }
+            }
+            double upd = 0.0;
+            for (int32_t i = 0; i < count; ++i) upd += double(probs[i].nnz);
+            best_rate = upd / T;
+        }
+        for (nomad_ctx* c : cur) nomad_destroy(c);
+    }
+    for (int32_t i = 0; i < count; ++i) strata_out[i] = Bsel[size_t(i)];
     if (updates_per_s) *updates_per_s = best_rate;
     return NOMAD_OK;
 }

@@ -1159,23 +1159,54 @@ int train_batch(nomad_ctx* const* cs, int32_t count, int64_t epochs, int32_t wav
         done_off[i] = ndone;
         ndone += c->P;
     }
-    // problems sorted by rounds, descending: those still running in a round
-    // are a prefix of this order
-    std::vector<int32_t> order(static_cast<size_t>(count));
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return rounds[x] > rounds[y]; });
-    std::vector<int32_t> boff(static_cast<size_t>(count) + 1, 0);
-    for (int32_t q = 0; q < count; ++q) boff[q + 1] = boff[q] + cs[order[q]]->P;
+    // strides: problem i runs s_i = ceil(B_i / Bref) of its rounds per ticket
+    // round, Bref the smallest strata count whose strides fit kBatchEntries
+    // (the largest count always does: every stride is then 1)
+    std::vector<int32_t> Bs(static_cast<size_t>(count));
+    for (int32_t i = 0; i < count; ++i) Bs[size_t(i)] = cs[i]->P;
+    std::sort(Bs.begin(), Bs.end());
+    int32_t Bref = Bs.back();
+    for (int32_t cand : Bs) {
+        int64_t sum = 0;
+        for (int32_t i = 0; i < count; ++i) sum += (cs[i]->P + cand - 1) / cand;
+        if (sum <= nomad_k::kBatchEntries) {
+            Bref = cand;
+            break;
+        }
+    }
+    struct Entry {
+        int32_t p, j, s;
+        int64_t ur;  // ticket rounds the entry runs: ceil((rounds_p - j) / s)
+    };
+    std::vector<Entry> ent;
+    for (int32_t i = 0; i < count; ++i) {
+        const int32_t si = (cs[i]->P + Bref - 1) / Bref;
+        for (int32_t j = 0; j < si; ++j) ent.push_back({i, j, si, (rounds[i] - j + si - 1) / si});
+    }
+    // sorted by ticket rounds descending (those still running in a ticket
+    // round are a prefix), then by the midpoint (2j + 1) / 2s of the entry's
+    // share of a ticket round, ascending
+    std::stable_sort(ent.begin(), ent.end(), [](const Entry& x, const Entry& y) {
+        if (x.ur != y.ur) return x.ur > y.ur;
+        return int64_t(2 * x.j + 1) * y.s < int64_t(2 * y.j + 1) * x.s;
+    });
+    const int32_t E = int32_t(ent.size());
+    std::vector<int32_t> order(static_cast<size_t>(E)), boff(static_cast<size_t>(E) + 1, 0);
+    for (int32_t q = 0; q < E; ++q) {
+        const Entry& x = ent[size_t(q)];
+        order[size_t(q)] = x.p | (x.j << 8) | (x.s << 18);
+        boff[size_t(q) + 1] = boff[size_t(q)] + cs[x.p]->P;
+    }
     std::vector<int64_t> seg_t0;
     std::vector<int32_t> seg_r0, seg_w;
     int64_t t = 0, r_prev = 0;
-    for (int32_t A = count; A >= 1; --A) {  // A problems running in rounds [r_prev, rounds of sorted A-1)
-        const int64_t r_end = rounds[order[A - 1]];
+    for (int32_t A = E; A >= 1; --A) {  // A entries running in ticket rounds [r_prev, that of sorted A-1)
+        const int64_t r_end = ent[size_t(A) - 1].ur;
         if (r_end <= r_prev) continue;
         seg_t0.push_back(t);
         seg_r0.push_back(int32_t(r_prev));
-        seg_w.push_back(boff[A]);
-        t += (r_end - r_prev) * boff[A];
+        seg_w.push_back(boff[size_t(A)]);
+        t += (r_end - r_prev) * boff[size_t(A)];
         r_prev = r_end;
     }
     const int64_t nitems = t;
@@ -1187,7 +1218,7 @@ int train_batch(nomad_ctx* const* cs, int32_t count, int64_t epochs, int32_t wav
     }));
     // default grid: one wave per tile of the widest round, up to the waves
     // that fit on the device at once (more would only wait)
-    int64_t want = waves > 0 ? waves : std::min<int64_t>(boff[count], int64_t(cus) * std::max(per_cu, 1) * 4);
+    int64_t want = waves > 0 ? waves : std::min<int64_t>(boff[size_t(E)], int64_t(cus) * std::max(per_cu, 1) * 4);
     const int blocks = int(std::max<int64_t>(1, std::min<int64_t>((want + 3) / 4, 65535)));
 
     struct Scratch {
@@ -1245,7 +1276,7 @@ int train_batch(nomad_ctx* const* cs, int32_t count, int64_t epochs, int32_t wav
     ba.ticket = w.ticket;
     ba.timed_out = w.timed_out;
     ba.nitems = nitems;
-    ba.nsorted = count;
+    ba.nsorted = E;
     ba.nseg = int32_t(nseg);
     ba.deadline = uint64_t(watchdog_seconds() * c0->wall_hz);
     NOMAD_HIP_TRY(hipEventRecord(c0->ev0, c0->stream));

@@ -109,22 +109,26 @@ struct SgdArgs {
     int32_t epoch;             // strat: the epoch; async: the launch index
 };
 
-// Batched stratified launch: at most kBatchMax problems per launch.
+// Batched stratified launch: at most kBatchMax problems per launch, and at
+// most kBatchEntries (problem, stride offset) entries: the sum over problems
+// of their rounds per ticket round.  order[q] packs entry q as problem
+// (bits 0-7) | offset j (bits 8-17) | stride s (bits 18-27), s <= kBatchEntries.
 constexpr int kBatchMax = 256;
+constexpr int kBatchEntries = 256;  // the LDS tables of 256 entries keep two workgroups per CU
 
 struct BatchArgs {
     const SgdArgs* __restrict__ probs;  // per problem; .epoch = its first epoch, .nblocks = its strata B
     const int64_t* seg_t0;    // nseg: first ticket of each run of equally wide rounds
-    const int32_t* seg_r0;    // nseg: first round of the run
-    const int32_t* seg_w;     // nseg: tiles per round in the run
-    const int32_t* boff;      // nsorted + 1: offset of each sorted problem's tiles in a round
-    const int32_t* order;     // nsorted: sorted position -> problem index
+    const int32_t* seg_r0;    // nseg: first ticket round of the run
+    const int32_t* seg_w;     // nseg: tiles per ticket round in the run
+    const int32_t* boff;      // nsorted + 1: offset of each sorted entry's tiles in a ticket round
+    const int32_t* order;     // nsorted: sorted position -> packed entry
     const int64_t* done_off;  // per problem: first of its B progress counters in done
     uint32_t* done;           // per (problem, row block): rounds completed (uncached)
     uint64_t* ticket;         // next ticket (uncached)
     uint32_t* timed_out;      // set when a wave gave up waiting (uncached)
     int64_t nitems;           // tickets in the launch
-    int32_t nsorted, nseg;
+    int32_t nsorted, nseg;    // entries (<= kBatchEntries), segments (<= nsorted)
     uint64_t deadline;        // wall_clock64 ticks a wave may run before it gives up waiting
 };
 

@@ -23,7 +23,7 @@
 
 using namespace nomad_dev;
 using nomad_k::BatchArgs;
-using nomad_k::kBatchMax;
+using nomad_k::kBatchEntries;
 using nomad_k::SgdArgs;
 using nomad_k::VisitDesc;
 
@@ -323,13 +323,20 @@ __global__ __launch_bounds__(64) void sgd_strat_kernel(SgdArgs a) {
 // two.  So tile (r, b) may start once both have finished, and it then sees
 // exactly the values the one-launch-per-sub-epoch schedule gives it.
 //
-// Tickets: round by round (all problems' tiles of round r, then round r + 1),
-// problems sorted by R_p descending so that the problems still running in a
-// round are a prefix of the sorted list.  A wave takes the next ticket, waits
-// for the two parent tiles (their tickets are smaller, so they were taken by
-// waves that are running), runs the tile and publishes it.  The smallest
-// unfinished ticket can always run, so the launch needs no co-residency and
-// finishes for any grid size.
+// Tickets: problems advance in proportion to their strata.  Problem p runs
+// s_p rounds per ticket round ("stride"; s_p = ceil(B_p / Bref), so that a
+// problem of many thin strata does not run on alone after the others have
+// finished): it appears as s_p entries (p, j), j < s_p, and entry (p, j) of
+// ticket round u is round u * s_p + j of p.  Entries are sorted by their
+// ticket rounds descending, so that those still running in a ticket round are
+// a prefix of the sorted list, and within equal counts by the midpoint of
+// their share of the ticket round, (j + 1/2) / s_p, so that the rounds of a
+// strided problem are spread between the other problems' tiles.  Round
+// u * s_p + j - 1 of p then always has smaller tickets than round u * s_p + j.
+// A wave takes the next ticket, waits for the two parent tiles (their tickets
+// are smaller, so they were taken by waves that are running), runs the tile
+// and publishes it.  The smallest unfinished ticket can always run, so the
+// launch needs no co-residency and finishes for any grid size.
 //
 // Publication: the next tile on a block may run on another XCD, whose L2 is
 // not this one's.  W and H rows are therefore read and written with
@@ -340,11 +347,11 @@ __global__ __launch_bounds__(64) void sgd_strat_kernel(SgdArgs a) {
 // with device-coherent loads.
 template <int S>
 __global__ __launch_bounds__(256) void sgd_batch_kernel(BatchArgs ba) {
-    __shared__ int64_t seg_t0[kBatchMax];
-    __shared__ int32_t seg_r0[kBatchMax];
-    __shared__ int32_t seg_w[kBatchMax];
-    __shared__ int32_t boff[kBatchMax + 1];
-    __shared__ int32_t order[kBatchMax];
+    __shared__ int64_t seg_t0[kBatchEntries];
+    __shared__ int32_t seg_r0[kBatchEntries];
+    __shared__ int32_t seg_w[kBatchEntries];
+    __shared__ int32_t boff[kBatchEntries + 1];
+    __shared__ int32_t order[kBatchEntries];
     __shared__ TileLds<S> tl[4];  // per wave (256-thread workgroups)
     const int Q = ba.nsorted, G = ba.nseg;
     for (int i = threadIdx.x; i <= Q; i += blockDim.x) {
@@ -365,15 +372,30 @@ __global__ __launch_bounds__(256) void sgd_batch_kernel(BatchArgs ba) {
         if (lane == 0) t = __hip_atomic_fetch_add(ba.ticket, uint64_t(1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         t = uniform64(t);
         if (t >= uint64_t(ba.nitems)) break;
-        // segment: rounds of constant width (the same problems still running)
-        int sg = 0;
-        while (sg + 1 < G && uint64_t(seg_t0[sg + 1]) <= t) ++sg;
+        // segment: ticket rounds of constant width (the same entries still
+        // running); the last segment with seg_t0 <= t
+        int sg = 0, sh = G;
+        while (sh - sg > 1) {
+            const int mid = (sg + sh) >> 1;
+            if (uint64_t(seg_t0[mid]) <= t)
+                sg = mid;
+            else
+                sh = mid;
+        }
         const uint64_t rel = t - uint64_t(seg_t0[sg]);
-        const int32_t r = seg_r0[sg] + int32_t(rel / uint32_t(seg_w[sg]));
+        const int32_t u = seg_r0[sg] + int32_t(rel / uint32_t(seg_w[sg]));
         const int32_t local = int32_t(rel % uint32_t(seg_w[sg]));
-        int q = 0;
-        while (q + 1 < Q && boff[q + 1] <= local) ++q;
-        const int p = __builtin_amdgcn_readfirstlane(order[q]);
+        int q = 0, qh = Q;  // the last entry with boff <= local
+        while (qh - q > 1) {
+            const int mid = (q + qh) >> 1;
+            if (boff[mid] <= local)
+                q = mid;
+            else
+                qh = mid;
+        }
+        const uint32_t ent = uint32_t(__builtin_amdgcn_readfirstlane(order[q]));
+        const int p = int(ent & 0xFFu);
+        const int32_t r = u * int32_t(ent >> 18) + int32_t((ent >> 8) & 0x3FFu);  // the problem's round
         const int b = __builtin_amdgcn_readfirstlane(local - boff[q]);
         const SgdArgs a = ba.probs[p];  // by value: its fields stay in registers across the row stores
         const int B = a.nblocks;
