@@ -30,8 +30,9 @@ at 100,000 x 20,000 with 10 M cells, k = 64), and `--workload sweep`: the batche
 hyper-parameter sweep -- 32 independent problems of uneven sizes (users ~
 logU(10k, 200k), items ~ logU(2k, 50k), ratings ~ logU(0.5M, 20M), k in
 {16, 32, 64, 128}, reg in {0.01, 0.05, 0.1, 0.5}), BF16 storage with
-stochastic rounding -- with each problem's strata count picked by the
-library's autotuner, then all problems trained in one persistent launch per
+stochastic rounding -- with each problem's strata count and row balance
+picked by the library's autotuner, then all problems trained in one
+persistent launch per
 step (one step = one epoch of every problem), beside the same epochs run one
 problem after another.
 """
@@ -241,20 +242,23 @@ def sweep(a, lib) -> dict:
     log(f"sweep: generated {len(probs)} problems, {total} training ratings in {time.perf_counter() - t0:.1f} s")
     strata, tune_s = [], time.perf_counter()
     fixed = [int(x) for x in a.strata.split(",")] if a.strata else []
+    bal = [x == "1" for x in a.balance.split(",")] if a.balance else []
     if fixed:
         strata = [fixed[i % len(fixed)] for i in range(len(probs))]
+        balanced = [bal[i % len(bal)] for i in range(len(probs))] if bal else [False] * len(probs)
     else:
         # the batch is tuned as a whole: the problems share the device
-        strata, rate = lib.autotune_batch([(tr, k, reg) for tr, te, k, reg in probs], storage="bf16",
-                                          rounding="stochastic", **hyp)
-        log(f"sweep: autotuned strata {sorted(set(strata))} ({rate / 1e9:.3f} G updates/s in the trial)")
+        strata, balanced, rate = lib.autotune_batch([(tr, k, reg) for tr, te, k, reg in probs], storage="bf16",
+                                                    rounding="stochastic", balance=True, tune_balance=True, **hyp)
+        log(f"sweep: autotuned strata {sorted(set(strata))}, {sum(balanced)} of {len(probs)} problems with "
+            f"balanced row blocks ({rate / 1e9:.3f} G updates/s in the trial)")
     tune_s = time.perf_counter() - tune_s
 
     def contexts():
         cs = []
         for i, (tr, te, k, reg) in enumerate(probs):
             c = lib.Context(tr.m, tr.n, k, storage="bf16", rounding="stochastic", mode="strat", nworkers=strata[i],
-                            seed=12345 + i, reg=reg, **hyp)
+                            seed=12345 + i, reg=reg, balance=balanced[i], **hyp)
             c.upload(tr, te)
             W0, H0 = lib.init_factors(tr.m, tr.n, k, 4, 12345 + i)
             c.set_factors(W0, H0)
@@ -301,7 +305,8 @@ def sweep(a, lib) -> dict:
         "data": "synthetic",
         "config": {
             "workload": f"batched strat sweep, {len(probs)} problems, one epoch of each per launch",
-            "problems": [dict(m=p[0].m, n=p[0].n, nnz=p[0].nnz, k=p[2], reg=p[3], B=b) for p, b in zip(probs, strata)],
+            "problems": [dict(m=p[0].m, n=p[0].n, nnz=p[0].nnz, k=p[2], reg=p[3], B=b, balanced=bl)
+                         for p, b, bl in zip(probs, strata, balanced)],
             "storage": "bf16", "rounding": "stochastic", **hyp, "scale": a.scale,
         },
         "roofline": {
@@ -492,6 +497,8 @@ def main() -> int:
     ap.add_argument("--problems", type=int, default=32, help="sweep: number of problems")
     ap.add_argument("--strata", default="", help="sweep: comma-separated strata per problem (cycled) instead of "
                                                    "the autotuner's choice")
+    ap.add_argument("--balance", default="", help="sweep, with --strata: comma-separated 0/1 per problem (cycled), "
+                                                 "1 = row blocks balanced by ratings (default all 0)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); multi-GPU token exchange is a design, "

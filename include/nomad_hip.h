@@ -50,8 +50,12 @@ extern "C" {
 #define NOMAD_ROUND_STOCHASTIC 1
 
 /* nomad_params.flags */
-#define NOMAD_FLAG_BALANCE 1 /* async: row blocks balanced by training ratings
-                                (nomad_row_blocks) instead of ceil(m / P) rows */
+#define NOMAD_FLAG_BALANCE 1 /* row blocks balanced by training ratings
+                                (nomad_row_blocks) instead of ceil(m / P) rows:
+                                async workers (with the last wave of a
+                                workgroup at reduced capacity), strat strata
+                                (capacities all 1; results stay bit-exact
+                                against the oracle given the same bounds) */
 #define NOMAD_FLAG_TIMING 2  /* async: collect the wall-clock counters [8..13]
                                 of nomad_get_counters (slows the kernel)    */
 
@@ -157,7 +161,8 @@ int nomad_train_batch(nomad_ctx* const* ctxs, int32_t count, int64_t epochs, int
  *               16} (the workgroup shape and the ring capacity follow from
  *               the worker count), with and without NOMAD_FLAG_BALANCE;
  *   strat mode: strata B in {1/4, 1/2, 1, 2, 4} x compute units (at most
- *               m), timed through nomad_train_batch with the default grid.
+ *               m), with and without NOMAD_FLAG_BALANCE, timed through
+ *               nomad_train_batch with the default grid.
  * params gives the problem and hyper-parameters; its nworkers and the
  * BALANCE flag are ignored.  Each trial trains a scratch context from
  * factors drawn from mt19937_64(params->seed), U(0, 1/sqrt(k)): async
@@ -178,20 +183,22 @@ int nomad_autotune(const nomad_params* params, const int64_t* colptr, const int3
                    int64_t nnz, double trial_seconds, nomad_tune_trial* best, nomad_tune_trial* trials,
                    int32_t max_trials, int32_t* ntrials);
 
-/* Batch autotuner (no reference counterpart): picks the strata count for a
- * batched sweep by timing whole batched epochs (nomad_train_batch), since
- * the problems then share the device and the best count differs from each
- * problem's own optimum.  Candidates: one strata count for every problem
- * in {64, 128, 256, 512, 1024} (each capped at the problem's rows).  Each
- * trial builds scratch strat contexts of the problems (params as given,
- * nworkers replaced; factors drawn from mt19937_64(params.seed), U(0,
- * 1/sqrt(k))) and times one batched epoch after a warm-up epoch.
- * A second stage times each problem's own epoch at the common count; the
- * problems within 30 % of the slowest, slowest first, then try 2x and 4x
- * that count one at a time, each kept only if it shortens the batched epoch
- * of all problems by at least 1 %.
- * strata_out[i] receives problem i's chosen count, *updates_per_s the
- * batch's rate with it.  count <= 256. */
+/* Batch autotuner (no reference counterpart): picks the strata count (and,
+ * with flags_out, the row balance) of each problem of a batched sweep by
+ * timing whole batched epochs (nomad_train_batch), since the problems then
+ * share the device and the best choice differs from each problem's own
+ * optimum.  Stage 1: one strata count for every problem in {64, 128, 256,
+ * 512, 1024} (each capped at the problem's rows).  Each trial builds
+ * scratch strat contexts of the problems (params as given, nworkers
+ * replaced; factors drawn from mt19937_64(params.seed), U(0, 1/sqrt(k)))
+ * and times one batched epoch after a warm-up epoch.  Stage 2 (greedy):
+ * each problem's own epoch is timed at the common count; the problems
+ * within 40 % of the slowest, slowest first, try 1x, 2x and 4x that count
+ * with NOMAD_FLAG_BALANCE as given and (only when flags_out is not NULL)
+ * toggled, and keep the move that shortens the batched epoch of all
+ * problems most, if by at least 1 %.  strata_out[i] receives problem i's
+ * chosen count, flags_out[i] (may be NULL) its params.flags with the chosen
+ * balance bit, *updates_per_s the batch's rate with them.  count <= 256. */
 typedef struct nomad_batch_problem {
     nomad_params params;  /* NOMAD_MODE_STRAT; one storage type and device for all */
     const int64_t* colptr;
@@ -201,7 +208,7 @@ typedef struct nomad_batch_problem {
 } nomad_batch_problem;
 
 int nomad_autotune_batch(const nomad_batch_problem* probs, int32_t count, int32_t* strata_out,
-                         double* updates_per_s);
+                         int32_t* flags_out, double* updates_per_s);
 
 /* Sum of squared errors over the training and test ratings, accumulated in
  * FP64 (the dot product runs in the storage's arithmetic type). */

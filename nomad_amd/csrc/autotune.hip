@@ -84,7 +84,10 @@ extern "C" int nomad_autotune(const nomad_params* params, const int64_t* colptr,
         int32_t last = -1;
         for (int num : {1, 2, 4, 8, 16}) {  // B = num / 4 x compute units
             const int32_t B = std::max(1, std::min(params->m, num * cus / 4));
-            if (B != last) cands.push_back({B, 0});
+            if (B != last) {
+                cands.push_back({B, NOMAD_FLAG_BALANCE});
+                cands.push_back({B, 0});
+            }
             last = B;
         }
     }
@@ -119,7 +122,7 @@ extern "C" int nomad_autotune(const nomad_params* params, const int64_t* colptr,
 }
 
 extern "C" int nomad_autotune_batch(const nomad_batch_problem* probs, int32_t count, int32_t* strata_out,
-                                    double* updates_per_s) {
+                                    int32_t* flags_out, double* updates_per_s) {
     using nomad_host::fail;
     if (!probs || count <= 0 || count > 256 || !strata_out)
         return fail(NOMAD_EINVAL, "nomad_autotune_batch: need 1..256 problems and an output array");
@@ -175,18 +178,26 @@ extern "C" int nomad_autotune_batch(const nomad_batch_problem* probs, int32_t co
     std::vector<int32_t> Bsel(static_cast<size_t>(count));
     for (int32_t i = 0; i < count; ++i) Bsel[size_t(i)] = std::min(best_B, probs[i].params.m);
 
+    std::vector<int32_t> Fsel(static_cast<size_t>(count));
+    for (int32_t i = 0; i < count; ++i) Fsel[size_t(i)] = probs[i].params.flags;
+
     // Second stage (greedy): a batched launch lasts at least as long as its
-    // slowest problem's chain of sub-epochs, and a sparse problem's chain is
-    // shorter with more strata; but every extra tile costs a ticket and a
-    // publication, so counts are raised one problem at a time.  Each
-    // problem's own epoch (a batched launch of one) is timed at the common
-    // count; the problems within 30 % of the slowest, slowest first, then try
-    // 2x and 4x the common count, each kept only if a batched epoch of all
-    // problems with it is at least 1 % shorter than the best so far (the
-    // shorter of two epochs after a warm-up epoch).
-    auto make = [&](int32_t i, int32_t B, nomad_ctx** out) -> int {
+    // slowest problem's chain of sub-epochs.  A sparse problem's chain is
+    // shorter with more strata, and a skewed one's with row blocks balanced
+    // by ratings (or, where balancing leaves blocks of many light rows that
+    // no longer fit the LDS staging, without); but every extra tile costs a
+    // ticket and a publication, so problems are changed one at a time.
+    // Each problem's own epoch (a batched launch of one) is timed at the
+    // common count; the problems within 40 % of the slowest, slowest first,
+    // then try 1x, 2x and 4x the common count with the balance flag as given
+    // and toggled (when flags_out is given; else as given only), and keep
+    // the move that makes the batched epoch of all problems shortest, if it
+    // is at least 1 % shorter than the best so far (each timing: the shorter
+    // of two epochs after a warm-up epoch).
+    auto make = [&](int32_t i, int32_t B, int32_t flags, nomad_ctx** out) -> int {
         nomad_params p = probs[i].params;
         p.nworkers = B;
+        p.flags = flags;
         *out = nullptr;
         int rc = nomad_create(&p, out);
         if (rc == NOMAD_OK) rc = nomad_upload_train(*out, probs[i].colptr, probs[i].rowidx, probs[i].val, probs[i].nnz);
@@ -209,7 +220,7 @@ extern "C" int nomad_autotune_batch(const nomad_batch_problem* probs, int32_t co
         std::vector<double> own(static_cast<size_t>(count), 0.0);
         int rc = NOMAD_OK;
         for (int32_t i = 0; i < count && rc == NOMAD_OK; ++i) {
-            rc = make(i, Bsel[size_t(i)], &cur[size_t(i)]);
+            rc = make(i, Bsel[size_t(i)], Fsel[size_t(i)], &cur[size_t(i)]);
             if (rc == NOMAD_OK) rc = timed(&cur[size_t(i)], 1, &own[size_t(i)]);
         }
         double T = 0.0;
@@ -218,29 +229,47 @@ extern "C" int nomad_autotune_batch(const nomad_batch_problem* probs, int32_t co
             const double own_max = *std::max_element(own.begin(), own.end());
             std::vector<int32_t> cand;
             for (int32_t i = 0; i < count; ++i)
-                if (own[size_t(i)] >= 0.7 * own_max) cand.push_back(i);
+                if (own[size_t(i)] >= 0.6 * own_max) cand.push_back(i);
             std::stable_sort(cand.begin(), cand.end(),
                              [&](int32_t x, int32_t y) { return own[size_t(x)] > own[size_t(y)]; });
             for (int32_t i : cand) {
-                for (int32_t f : {2, 4}) {
-                    const int32_t B = std::min(best_B * f, probs[i].params.m);
-                    if (B <= Bsel[size_t(i)]) break;
-                    nomad_ctx* nc = nullptr;
-                    if (make(i, B, &nc) != NOMAD_OK) {
-                        nomad_destroy(nc);
-                        break;
-                    }
-                    nomad_ctx* old = cur[size_t(i)];
-                    cur[size_t(i)] = nc;
-                    double t = 0.0;
-                    if (timed(cur.data(), count, &t) == NOMAD_OK && t > 0.0 && t < 0.99 * T) {
-                        nomad_destroy(old);
-                        T = t;
-                        Bsel[size_t(i)] = B;
-                    } else {
+                const int32_t f0 = Fsel[size_t(i)];
+                nomad_ctx* best_ctx = nullptr;
+                double best_t = 0.99 * T;
+                int32_t best_b = 0, best_f = 0;
+                for (int32_t mult : {1, 2, 4}) {
+                    const int32_t B = std::min(best_B * mult, probs[i].params.m);
+                    if (mult > 1 && B <= std::min(best_B * mult / 2, probs[i].params.m)) break;  // capped at m
+                    for (int32_t f : {f0, f0 ^ NOMAD_FLAG_BALANCE}) {
+                        if (f != f0 && !flags_out) continue;
+                        if (B == Bsel[size_t(i)] && f == f0) continue;  // the current configuration
+                        nomad_ctx* nc = nullptr;
+                        if (make(i, B, f, &nc) != NOMAD_OK) {
+                            nomad_destroy(nc);
+                            continue;
+                        }
+                        nomad_ctx* old = cur[size_t(i)];
+                        cur[size_t(i)] = nc;
+                        double t = 0.0;
+                        const bool ok = timed(cur.data(), count, &t) == NOMAD_OK && t > 0.0;
                         cur[size_t(i)] = old;
-                        nomad_destroy(nc);
+                        if (ok && t < best_t) {
+                            nomad_destroy(best_ctx);
+                            best_ctx = nc;
+                            best_t = t;
+                            best_b = B;
+                            best_f = f;
+                        } else {
+                            nomad_destroy(nc);
+                        }
                     }
+                }
+                if (best_ctx) {
+                    nomad_destroy(cur[size_t(i)]);
+                    cur[size_t(i)] = best_ctx;
+                    T = best_t;
+                    Bsel[size_t(i)] = best_b;
+                    Fsel[size_t(i)] = best_f;
                 }
             }
             double upd = 0.0;
@@ -249,7 +278,10 @@ extern "C" int nomad_autotune_batch(const nomad_batch_problem* probs, int32_t co
         }
         for (nomad_ctx* c : cur) nomad_destroy(c);
     }
-    for (int32_t i = 0; i < count; ++i) strata_out[i] = Bsel[size_t(i)];
+    for (int32_t i = 0; i < count; ++i) {
+        strata_out[i] = Bsel[size_t(i)];
+        if (flags_out) flags_out[i] = Fsel[size_t(i)];
+    }
     if (updates_per_s) *updates_per_s = best_rate;
     return NOMAD_OK;
 }

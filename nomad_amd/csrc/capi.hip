@@ -291,8 +291,8 @@ struct nomad_ctx {
     int rpb = 0, cpb = 0, ldf = 0;
     int units = 0;        // async: workgroups (route units)
     int nw = 16;          // async: waves (row blocks) per workgroup
-    std::vector<int32_t> h_wbound;  // async: first row of each row block, P + 1 entries
-    std::vector<int32_t> row_unit;  // async: workgroup owning each row
+    std::vector<int32_t> h_wbound;  // first row of each row block, P + 1 entries (async workers, strat strata)
+    std::vector<int32_t> row_unit;  // async: workgroup owning each row; strat: row block of each row
     int32_t* d_wbound = nullptr;
     size_t esz = 4, rvsz = 4;
     uint32_t sr_seed = 0;
@@ -460,26 +460,45 @@ int upload_csc(nomad_ctx* c, DevCsc& d, const int64_t* colptr, const int32_t* ro
     return NOMAD_OK;
 }
 
-// Tile lists of the stratified schedule: row block b = rows [b * rpb,
-// (b + 1) * rpb), column block cb = columns [cb * cpb, (cb + 1) * cpb).  The
+// Tile lists of the stratified schedule: row block b = rows [h_wbound[b],
+// h_wbound[b + 1]), column block cb = columns [cb * cpb, (cb + 1) * cpb).  The
 // visits of row block b are listed by ascending column, so tile (b, cb) is
 // the contiguous run of them whose column lies in column block cb.  One pass
 // over the ratings; a final sentinel visit (len 0) lets a tile loop read one
 // descriptor ahead without a bounds test.
 int ensure_tiles(nomad_ctx* c) {
     if (c->tiles_ready) return NOMAD_OK;
-    const int32_t n = c->p.n, B = c->P;
+    const int32_t n = c->p.n, B = c->P, m = c->p.m;
+    // Row blocks: ceil(m / B) rows each, or with NOMAD_FLAG_BALANCE the
+    // contiguous split whose largest training-rating count is smallest
+    // (nomad_row_blocks): a sub-epoch waits for its heaviest tile, so blocks
+    // of equal ratings shorten it.
+    c->h_wbound.assign(size_t(B) + 1, m);
+    if (c->p.flags & NOMAD_FLAG_BALANCE) {
+        const int brc = nomad_row_blocks(c->h_rowidx.data(), int64_t(c->h_rowidx.size()), m, B, nullptr,
+                                         c->h_wbound.data());
+        if (brc) return brc;  // (its message is set)
+    } else {
+        for (int b = 0; b <= B; ++b) c->h_wbound[size_t(b)] = int32_t(std::min<int64_t>(int64_t(b) * c->rpb, m));
+    }
+    int32_t maxrows = 0;
+    c->row_unit.assign(size_t(m), 0);
+    for (int32_t b = 0; b < B; ++b) {
+        maxrows = std::max(maxrows, c->h_wbound[size_t(b) + 1] - c->h_wbound[size_t(b)]);
+        for (int32_t r = c->h_wbound[size_t(b)]; r < c->h_wbound[size_t(b) + 1]; ++r) c->row_unit[size_t(r)] = b;
+    }
     // a row block is addressed through one buffer resource (32-bit range)
-    if (int64_t(c->rpb) * c->ldf * int64_t(c->esz) >= (int64_t(1) << 31))
+    if (int64_t(maxrows) * c->ldf * int64_t(c->esz) >= (int64_t(1) << 31))
         return fail(NOMAD_EINVAL, "strat mode: a row block exceeds 2 GiB of W; use more strata");
+    const std::vector<int32_t>& blk = c->row_unit;
     std::vector<std::vector<VisitDesc>> per(static_cast<size_t>(B));
     for (int32_t j = 0; j < n; ++j) {
         const int64_t c0 = c->h_colptr[j], c1 = c->h_colptr[j + 1];
         int64_t e = c0;
         while (e < c1) {
-            const int32_t b = c->h_rowidx[e] / c->rpb;
+            const int32_t b = blk[size_t(c->h_rowidx[e])];
             int64_t f = e + 1;
-            while (f < c1 && c->h_rowidx[f] / c->rpb == b) ++f;
+            while (f < c1 && blk[size_t(c->h_rowidx[f])] == b) ++f;
             per[b].push_back(VisitDesc{e, int32_t(f - e), j});
             e = f;
         }
@@ -508,9 +527,9 @@ int ensure_tiles(nomad_ctx* c) {
     std::vector<int32_t> trows;
     std::vector<uint16_t> tslot(static_cast<size_t>(std::max<int64_t>(nnz, 1)), 0);
     {
-        std::vector<int32_t> mark(static_cast<size_t>(c->rpb), -1), rows_t;
+        std::vector<int32_t> mark(static_cast<size_t>(std::max(maxrows, 1)), -1), rows_t;
         for (int32_t t = 0; t < B * B; ++t) {
-            const int32_t rlo = (t / B) * c->rpb;
+            const int32_t rlo = c->h_wbound[size_t(t / B)];
             trp[size_t(t)] = int64_t(trows.size());
             rows_t.clear();
             for (int64_t v = tptr[size_t(t)]; v < tptr[size_t(t) + 1]; ++v)
@@ -559,7 +578,10 @@ int ensure_tiles(nomad_ctx* c) {
     dev_free(c->d_tslot);
     dev_free(c->d_tm_row);
     dev_free(c->d_tm_val);
+    dev_free(c->d_wbound);
     int rc;
+    if ((rc = dev_alloc(&c->d_wbound, c->h_wbound.size()))) return rc;
+    NOMAD_HIP_TRY(hipMemcpy(c->d_wbound, c->h_wbound.data(), c->h_wbound.size() * 4, hipMemcpyHostToDevice));
     if ((rc = dev_alloc(&c->d_tile_ptr, tptr.size())) || (rc = dev_alloc(&c->d_tile_visits, vis.size())) ||
         (rc = dev_alloc(&c->d_trow_ptr, trp.size())) || (rc = dev_alloc(&c->d_trows, std::max<size_t>(trows.size(), 1))) ||
         (rc = dev_alloc(&c->d_tslot, ntm)) || (rc = dev_alloc(&c->d_tm_row, ntm)) ||
@@ -837,6 +859,7 @@ SgdArgs base_args(nomad_ctx* c) {
     a.val = c->tr.val;
     a.steps = c->d_steps;
     if (c->p.mode == NOMAD_MODE_STRAT) {
+        a.wbound = c->d_wbound;
         a.tile_ptr = c->d_tile_ptr;
         a.visits = c->d_tile_visits;
         a.rowidx = c->d_tm_row;
@@ -1080,7 +1103,7 @@ void strat_stats(const nomad_ctx* c, int64_t epochs, double seconds, nomad_train
     for (int32_t j = 0; j < c->p.n; ++j) {
         int32_t last = -1;
         for (int64_t e = c->h_colptr[j]; e < c->h_colptr[j + 1]; ++e) {
-            const int32_t b = c->h_rowidx[e] / c->rpb;
+            const int32_t b = c->row_unit[size_t(c->h_rowidx[e])];
             if (b != last) {
                 ++vis;
                 last = b;

@@ -81,6 +81,7 @@ struct SgdArgs {
 
     // async schedule (two-level: workgroups of waves)
     const int32_t* wbound;     // nblocks * waves + 1 first rows of the worker row blocks
+                               // (strat: nblocks + 1 first rows of the strata's row blocks)
     const int64_t* vptr;       // n + 1 offsets into route
     const RouteDesc* route;    // per column, its hops in route order
     int32_t* vcnt;             // per (column, workgroup) pair: visits so far (the step-size counter)

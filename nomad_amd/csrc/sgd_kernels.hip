@@ -105,7 +105,7 @@ __device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep,
     E* __restrict__ H = static_cast<E*>(a.H);
     const gptr<R> rv = gbl(static_cast<const R*>(a.val));
     const int npairs = a.ldf >> 1;
-    const int32_t rlo = b * a.rpb, rhi = min(a.m, (b + 1) * a.rpb);
+    const int32_t rlo = gbl(a.wbound)[b], rhi = gbl(a.wbound)[b + 1];
     const RowBlock<S> rb = row_block<S>(a, rlo, rhi, lane);
     const uint32_t lkey = sr_lane_key(a.sr_seed, uint32_t(lane));
     const int32_t staged = nrows * npairs;

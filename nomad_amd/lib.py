@@ -120,7 +120,7 @@ HIP_SYMBOLS = {
     "nomad_train_batch": (C.c_int, [C.POINTER(_P), C.c_int32, C.c_int64, C.c_int32, C.POINTER(TrainStats)]),
     "nomad_autotune": (C.c_int, [C.POINTER(Params), _i64p, _i32p, _f64p, C.c_int64, C.c_double, C.POINTER(TuneTrial),
                                  C.POINTER(TuneTrial), C.c_int32, _i32p]),
-    "nomad_autotune_batch": (C.c_int, [C.POINTER(BatchProblem), C.c_int32, _i32p, C.POINTER(C.c_double)]),
+    "nomad_autotune_batch": (C.c_int, [C.POINTER(BatchProblem), C.c_int32, _i32p, _i32p, C.POINTER(C.c_double)]),
     "nomad_obj_create": (C.c_int, [C.POINTER(ObjParams), _i64p, _i32p, _f64p, C.c_int64, C.POINTER(_P)]),
     "nomad_obj_eval": (C.c_int, [_P, _P, _P, C.c_double, _P, _P, C.POINTER(C.c_double), _P]),
     "nomad_obj_destroy": (C.c_int, [_P]),
@@ -603,9 +603,12 @@ def autotune(train: Csc, k: int, *, mode: str = "async", storage: str = "fp32", 
 
 
 def autotune_batch(problems, *, storage: str = "bf16", rounding: str = "stochastic", lrate: float = 0.005,
-                   drate: float = 0.05, seed: int = 12345, device: int = 0):
+                   drate: float = 0.05, seed: int = 12345, device: int = 0, balance: bool = False,
+                   tune_balance: bool = False):
     """nomad_autotune_batch for [(train Csc, k, reg), ...]: returns (strata per
-    problem, the batch's updates/s with them)."""
+    problem, balanced row blocks per problem (bools), the batch's updates/s
+    with them).  balance: the starting row balance (NOMAD_FLAG_BALANCE) of
+    every problem; tune_balance: let the tuner toggle it per problem."""
     lib = hip()
     arr = (BatchProblem * len(problems))()
     keep = []
@@ -617,6 +620,7 @@ def autotune_batch(problems, *, storage: str = "bf16", rounding: str = "stochast
         p.rounding = {"nearest": ROUND_NEAREST, "stochastic": ROUND_STOCHASTIC}[rounding]
         p.device = device
         p.lrate, p.drate, p.reg, p.seed = lrate, drate, reg, seed + i
+        p.flags = FLAG_BALANCE if balance else 0
         cp = np.ascontiguousarray(tr.colptr, np.int64)
         ri = np.ascontiguousarray(tr.rowidx, np.int32)
         va = np.ascontiguousarray(tr.val, np.float64)
@@ -624,8 +628,10 @@ def autotune_batch(problems, *, storage: str = "bf16", rounding: str = "stochast
         arr[i].colptr, arr[i].rowidx, arr[i].val, arr[i].nnz = (_ptr(cp, C.c_int64), _ptr(ri, C.c_int32),
                                                                 _ptr(va, C.c_double), tr.nnz)
     out = np.zeros(len(problems), np.int32)
+    flags = np.full(len(problems), FLAG_BALANCE if balance else 0, np.int32)
     rate = C.c_double()
-    rc = lib.nomad_autotune_batch(arr, len(problems), _ptr(out, C.c_int32), C.byref(rate))
+    rc = lib.nomad_autotune_batch(arr, len(problems), _ptr(out, C.c_int32),
+                                  _ptr(flags, C.c_int32) if tune_balance else None, C.byref(rate))
     if rc != 0:
         raise NomadError(rc, lib.nomad_last_error().decode())
-    return out.tolist(), rate.value
+    return out.tolist(), [bool(f & FLAG_BALANCE) for f in flags.tolist()], rate.value

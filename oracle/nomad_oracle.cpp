@@ -288,10 +288,9 @@ double orc_step(int storage, double lrate, double drate, int count) {
 // ---------------------------------------------------------------------------
 template <class A>
 static void strat_replay(int arith, int m, int n, int k, const int64_t* colptr, const int32_t* rowidx,
-                         const double* val, std::vector<A>& W, std::vector<A>& H, int B, int epoch0,
-                         int nepochs, double lrate, double drate, double reg, int storage) {
+                         const double* val, std::vector<A>& W, std::vector<A>& H, int B, const int32_t* rbound,
+                         int epoch0, int nepochs, double lrate, double drate, double reg, int storage) {
     const int ldf = ldf_of(k);
-    const int rpb = m / B + ((m % B) > 0 ? 1 : 0);
     const int cpb = n / B + ((n % B) > 0 ? 1 : 0);
     const A regA = storage == ORC_FP64 ? A(reg) : A(float(reg));
     for (int e = 0; e < nepochs; ++e) {
@@ -299,7 +298,7 @@ static void strat_replay(int arith, int m, int n, int k, const int64_t* colptr, 
         for (int s = 0; s < B; ++s) {
             for (int b = 0; b < B; ++b) {
                 const int cb = (b + s) % B;
-                const int rlo = b * rpb, rhi = std::min(m, rlo + rpb);
+                const int rlo = rbound[b], rhi = rbound[b + 1];
                 const int clo = cb * cpb, chi = std::min(n, clo + cpb);
                 for (int j = clo; j < chi; ++j) {
                     A* h = &H[int64_t(j) * ldf];
@@ -324,11 +323,10 @@ static void strat_replay(int arith, int m, int n, int k, const int64_t* colptr, 
 // the visit ends; w is rounded after each update.
 static void strat_replay_bf16(int rounding, uint32_t sr_seed, int m, int n, int k, const int64_t* colptr,
                               const int32_t* rowidx, const double* val, std::vector<uint16_t>& W,
-                              std::vector<uint16_t>& H, int B, int epoch0, int nepochs, double lrate,
-                              double drate, double reg) {
+                              std::vector<uint16_t>& H, int B, const int32_t* rbound, int epoch0, int nepochs,
+                              double lrate, double drate, double reg) {
     const int ldf = ldf_of(k);
     const int npairs = ldf / 2;
-    const int rpb = m / B + ((m % B) > 0 ? 1 : 0);
     const int cpb = n / B + ((n % B) > 0 ? 1 : 0);
     const float regf = float(reg);
     std::vector<float> h(ldf), w(ldf);
@@ -352,7 +350,7 @@ static void strat_replay_bf16(int rounding, uint32_t sr_seed, int m, int n, int 
         for (int s = 0; s < B; ++s) {
             for (int b = 0; b < B; ++b) {
                 const int cb = (b + s) % B;
-                const int rlo = b * rpb, rhi = std::min(m, rlo + rpb);
+                const int rlo = rbound[b], rhi = rbound[b + 1];
                 const int clo = cb * cpb, chi = std::min(n, clo + cpb);
                 for (int j = clo; j < chi; ++j) {
                     bool any = false;
@@ -389,9 +387,17 @@ void orc_bf16_store_pair(float x0, float x1, int rounding, uint32_t seed, uint32
 
 int64_t orc_sgd_strat(int storage, int arith, int rounding, uint32_t sr_seed, int m, int n, int k,
                       const int64_t* colptr, const int32_t* rowidx, const double* val, double* W, double* H,
-                      int B, int epoch0, int nepochs, double lrate, double drate, double reg) {
+                      int B, const int32_t* row_bounds, int epoch0, int nepochs, double lrate, double drate,
+                      double reg) {
     if (m <= 0 || n <= 0 || k <= 0 || B <= 0 || nepochs < 0) return -1;
     const int ldf = ldf_of(k);
+    // row block b = rows [rb[b], rb[b + 1]): the caller's bounds, or ceil(m / B) rows each
+    std::vector<int32_t> rb(size_t(B) + 1);
+    const int rpb = m / B + ((m % B) > 0 ? 1 : 0);
+    for (int b = 0; b <= B; ++b) rb[size_t(b)] = row_bounds ? row_bounds[b] : std::min(m, b * rpb);
+    if (rb[0] != 0 || rb[size_t(B)] != m) return -1;
+    for (int b = 0; b < B; ++b)
+        if (rb[size_t(b) + 1] < rb[size_t(b)]) return -1;
     if (storage == ORC_FP32 || storage == ORC_FP64) {
         auto run = [&](auto zero) {
             using A = decltype(zero);
@@ -400,8 +406,8 @@ int64_t orc_sgd_strat(int storage, int arith, int rounding, uint32_t sr_seed, in
                 for (int d = 0; d < k; ++d) w[i * ldf + d] = A(W[i * k + d]);
             for (int64_t j = 0; j < n; ++j)
                 for (int d = 0; d < k; ++d) h[j * ldf + d] = A(H[j * k + d]);
-            strat_replay<A>(arith, m, n, k, colptr, rowidx, val, w, h, B, epoch0, nepochs, lrate, drate, reg,
-                            storage);
+            strat_replay<A>(arith, m, n, k, colptr, rowidx, val, w, h, B, rb.data(), epoch0, nepochs, lrate, drate,
+                            reg, storage);
             for (int64_t i = 0; i < m; ++i)
                 for (int d = 0; d < k; ++d) W[i * k + d] = double(w[i * ldf + d]);
             for (int64_t j = 0; j < n; ++j)
@@ -417,8 +423,8 @@ int64_t orc_sgd_strat(int storage, int arith, int rounding, uint32_t sr_seed, in
             for (int d = 0; d < k; ++d) w[i * ldf + d] = bf16_rne(float(W[i * k + d]));
         for (int64_t j = 0; j < n; ++j)
             for (int d = 0; d < k; ++d) h[j * ldf + d] = bf16_rne(float(H[j * k + d]));
-        strat_replay_bf16(rounding, sr_seed, m, n, k, colptr, rowidx, val, w, h, B, epoch0, nepochs, lrate, drate,
-                          reg);
+        strat_replay_bf16(rounding, sr_seed, m, n, k, colptr, rowidx, val, w, h, B, rb.data(), epoch0, nepochs,
+                          lrate, drate, reg);
         for (int64_t i = 0; i < m; ++i)
             for (int d = 0; d < k; ++d) W[i * k + d] = bf16_to_f(w[i * ldf + d]);
         for (int64_t j = 0; j < n; ++j)

@@ -53,13 +53,16 @@ double orc_step(int storage, double lrate, double drate, int count);
  * Global CSC (rows ascending in a column).  W (m x k), H (n x k) are double
  * on input/output and held in `storage` precision during the replay.
  * Schedule strat(B): for sub-epoch s, tiles (b, (b+s) % B) with row blocks of
- * ceil(m/B) rows and column blocks of ceil(n/B) columns; inside a tile,
+ * ceil(m/B) rows, or rows [row_bounds[b], row_bounds[b+1]) when row_bounds
+ * (B + 1 entries, 0 = first <= ... <= last = m) is given, and column blocks
+ * of ceil(n/B) columns; inside a tile,
  * columns ascending, rows ascending.  B = 1 is the reference's one-thread
  * order (columns 0..n-1 cyclically).  The step of epoch e uses count
  * epoch0 + e.  Returns the number of updates, or a negative error. */
 int64_t orc_sgd_strat(int storage, int arith, int rounding, uint32_t sr_seed, int m, int n, int k,
                       const int64_t* colptr, const int32_t* rowidx, const double* val, double* W, double* H,
-                      int B, int epoch0, int nepochs, double lrate, double drate, double reg);
+                      int B, const int32_t* row_bounds, int epoch0, int nepochs, double lrate, double drate,
+                      double reg);
 
 /* --- one BF16 store of an element pair, as the kernels do it -------------- */
 void orc_bf16_store_pair(float x0, float x1, int rounding, uint32_t seed, uint32_t a, uint32_t b, uint32_t lane,

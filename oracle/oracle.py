@@ -32,7 +32,7 @@ _SYMS = {
     "orc_init_perms": (C.c_int, [C.c_int, C.c_int, C.c_uint64, _i32p]),
     "orc_step": (C.c_double, [C.c_int, C.c_double, C.c_double, C.c_int]),
     "orc_sgd_strat": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_int,
-                                  _i64p, _i32p, _f64p, _f64p, _f64p, C.c_int, C.c_int, C.c_int,
+                                  _i64p, _i32p, _f64p, _f64p, _f64p, C.c_int, _i32p, C.c_int, C.c_int,
                                   C.c_double, C.c_double, C.c_double]),
     "orc_sgd_async": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _i64p, _i32p, _f64p, _f64p, _f64p,
                                 C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64,
@@ -121,14 +121,18 @@ def step(storage, lrate, drate, count):
 
 
 def sgd_strat(a, W, H, *, storage=FP32, arith=ARITH_REF, rounding=ROUND_NEAREST, sr_seed=0, B=1, epoch0=0,
-              epochs=1, lrate=0.001, drate=0.1, reg=0.05):
+              epochs=1, lrate=0.001, drate=0.1, reg=0.05, row_bounds=None):
+    """row_bounds: B + 1 first rows of the row blocks (None: ceil(m / B) rows each)."""
     colptr, rowidx, val = _csc(a)
     W = np.array(W, np.float64, copy=True, order="C")
     H = np.array(H, np.float64, copy=True, order="C")
     k = W.shape[1]
+    rb = None if row_bounds is None else np.ascontiguousarray(row_bounds, np.int32)
+    if rb is not None and rb.shape != (B + 1,):
+        raise ValueError("row_bounds needs B + 1 entries")
     r = lib().orc_sgd_strat(storage, arith, rounding, sr_seed, a.m, a.n, k, _p(colptr, C.c_int64),
                             _p(rowidx, C.c_int32), _p(val, C.c_double), _p(W, C.c_double), _p(H, C.c_double),
-                            B, epoch0, epochs, lrate, drate, reg)
+                            B, None if rb is None else _p(rb, C.c_int32), epoch0, epochs, lrate, drate, reg)
     if r < 0:
         raise RuntimeError("orc_sgd_strat failed")
     return W, H

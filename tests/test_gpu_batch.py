@@ -26,10 +26,10 @@ def _problems(gpu, count, scale, first=0):
     return out
 
 
-def _ctx(gpu, tr, te, k, reg, B, storage, seed):
+def _ctx(gpu, tr, te, k, reg, B, storage, seed, balance=False):
     rounding = "stochastic" if storage == "bf16" else "nearest"
     c = gpu.Context(tr.m, tr.n, k, storage=storage, rounding=rounding, mode="strat", nworkers=B, seed=seed,
-                    lrate=0.002, drate=0.05, reg=reg)
+                    lrate=0.002, drate=0.05, reg=reg, balance=balance)
     c.upload(tr, te)
     W0, H0 = gpu.init_factors(tr.m, tr.n, k, 2, 1000 + seed)
     c.set_factors(W0, H0)
@@ -157,17 +157,19 @@ def test_autotune_picks_a_valid_fastest_config(gpu, small, mode):
 
 def test_autotune_batch_picks_valid_strata_counts(gpu):
     """Every chosen count is a candidate (a common count in 64..1024, or 2x
-    or 4x it for a slow problem) capped at the problem's rows."""
+    or 4x it for a slow problem) capped at the problem's rows; the chosen
+    counts and row balances train."""
     probs = _problems(gpu, 4, 0.01, first=20)
-    strata, rate = gpu.autotune_batch([(tr, k, reg) for tr, te, k, reg in probs])
-    print(f"[autotune batch] strata {strata}, {rate / 1e9:.3f} G updates/s", flush=True)
+    strata, balanced, rate = gpu.autotune_batch([(tr, k, reg) for tr, te, k, reg in probs], tune_balance=True)
+    print(f"[autotune batch] strata {strata}, balanced {balanced}, {rate / 1e9:.3f} G updates/s", flush=True)
     assert rate > 0
     uncapped = [B for (tr, te, k, reg), B in zip(probs, strata) if B != tr.m]
     common = min(uncapped) if uncapped else 0
     for (tr, te, k, reg), B in zip(probs, strata):
         assert 1 <= B <= tr.m
         assert B == tr.m or (B in (64, 128, 256, 512, 1024, 2048, 4096) and B in (common, 2 * common, 4 * common))
-    cs = [_ctx(gpu, tr, te, k, reg, B, "bf16", seed=i) for i, ((tr, te, k, reg), B) in enumerate(zip(probs, strata))]
+    cs = [_ctx(gpu, tr, te, k, reg, B, "bf16", seed=i, balance=bl)
+          for i, ((tr, te, k, reg), B, bl) in enumerate(zip(probs, strata, balanced))]
     st = gpu.train_batch(cs, 1)
     assert all(s.updates == p[0].nnz for s, p in zip(st, probs))
     for c in cs:
@@ -212,6 +214,50 @@ def test_batch_k64_k128_B256_vs_oracle(gpu, storage):
         c.close()
 
 
+@pytest.mark.parametrize("storage", ["fp32", "bf16", "fp64"])
+def test_balanced_strata_vs_oracle(gpu, storage):
+    """NOMAD_FLAG_BALANCE in strat mode: the row blocks are the balanced split
+    of nomad_row_blocks.  A single run and a batched run (beside an
+    unbalanced problem) are bit-identical to the oracle's replay with those
+    row bounds, on problems whose balanced split differs from ceil(m / B)."""
+    probs = _problems(gpu, 3, 0.01, first=21)
+    rounding = "stochastic" if storage == "bf16" else "nearest"
+    spec = [(16, True), (7, False), (32, True)]
+
+    def make(i):
+        tr, te, k, reg = probs[i]
+        B, bal = spec[i]
+        c = gpu.Context(tr.m, tr.n, k, storage=storage, rounding=rounding, mode="strat", nworkers=B, seed=i,
+                        lrate=0.002, drate=0.05, reg=reg, balance=bal)
+        c.upload(tr, te)
+        W0, H0 = gpu.init_factors(tr.m, tr.n, k, 2, 1000 + i)
+        c.set_factors(W0, H0)
+        return c
+
+    batch = [make(i) for i in range(3)]
+    gpu.train_batch(batch, 2)
+    for i, (tr, te, k, reg) in enumerate(probs):
+        B, bal = spec[i]
+        rb = gpu.row_blocks(tr.rowidx, tr.m, B) if bal else None
+        if bal:
+            eq = np.minimum(np.arange(B + 1) * -(-tr.m // B), tr.m)
+            assert not np.array_equal(rb, eq), "the balanced split must differ from the default one"
+        single = make(i)
+        single.train(epochs=2)
+        W0, H0 = gpu.init_factors(tr.m, tr.n, k, 2, 1000 + i)
+        ost = {"fp32": O.FP32, "bf16": O.BF16, "fp64": O.FP64}[storage]
+        rnd = O.ROUND_STOCHASTIC if storage == "bf16" else O.ROUND_NEAREST
+        Wo, Ho = O.sgd_strat(tr, W0, H0, storage=ost, arith=O.ARITH_GPU, rounding=rnd, sr_seed=_sr_seed(i), B=B,
+                             epochs=2, lrate=0.002, drate=0.05, reg=reg, row_bounds=rb)
+        print(f"[balanced {storage}] problem {i}: m={tr.m} n={tr.n} nnz={tr.nnz} k={k} B={B} balanced={bal}",
+              flush=True)
+        for c, name in ((single, "single"), (batch[i], "batched")):
+            Wg, Hg = c.get_factors()
+            assert np.array_equal(Wg, Wo) and np.array_equal(Hg, Ho), (name, i, np.abs(Wg - Wo).max())
+        single.close()
+        batch[i].close()
+
+
 def test_batch_sweep_full_size_autotuned(gpu):
     """The full batched sweep: 32 problems at full size (about 161 M ratings),
     the batch autotuner's common strata count, one batched epoch bit-identical
@@ -219,14 +265,17 @@ def test_batch_sweep_full_size_autotuned(gpu):
     probs = _problems(gpu, 32, 1.0)
     total = sum(p[0].nnz for p in probs)
     print(f"[sweep full] 32 problems, {total} training ratings", flush=True)
-    strata, rate = gpu.autotune_batch([(tr, k, reg) for tr, te, k, reg in probs])
-    print(f"[sweep full] autotuned strata {sorted(set(strata))}, {rate / 1e9:.3f} G updates/s", flush=True)
-    batch = [_ctx(gpu, tr, te, k, reg, B, "bf16", seed=i) for i, ((tr, te, k, reg), B) in enumerate(zip(probs, strata))]
+    strata, balanced, rate = gpu.autotune_batch([(tr, k, reg) for tr, te, k, reg in probs], balance=True,
+                                                tune_balance=True)
+    print(f"[sweep full] autotuned strata {sorted(set(strata))}, {sum(balanced)} balanced, {rate / 1e9:.3f} G "
+          f"updates/s", flush=True)
+    batch = [_ctx(gpu, tr, te, k, reg, B, "bf16", seed=i, balance=bl)
+             for i, ((tr, te, k, reg), B, bl) in enumerate(zip(probs, strata, balanced))]
     st = gpu.train_batch(batch, 1)
     print(f"[sweep full] batched epoch {st[0].train_seconds * 1e3:.1f} ms", flush=True)
     t_seq = 0.0
-    for i, ((tr, te, k, reg), B) in enumerate(zip(probs, strata)):
-        a = _ctx(gpu, tr, te, k, reg, B, "bf16", seed=i)
+    for i, ((tr, te, k, reg), B, bl) in enumerate(zip(probs, strata, balanced)):
+        a = _ctx(gpu, tr, te, k, reg, B, "bf16", seed=i, balance=bl)
         t_seq += a.train(epochs=1).train_seconds
         Wa, Ha = a.get_factors()
         Wb, Hb = batch[i].get_factors()

@@ -215,6 +215,47 @@ def test_strat_tiles_commute():
     assert np.allclose(Wo, W, rtol=1e-12, atol=1e-14) and np.allclose(Ho, H, rtol=1e-12, atol=1e-14)
 
 
+def test_strat_row_bounds():
+    """Row blocks given as bounds (uneven, one of them empty): the oracle
+    equals a numpy replay of the same tile sequence with those blocks; the
+    ceil(m / B) bounds give the default split bit for bit; malformed bounds
+    are rejected."""
+    tr, _ = lib.synth(lib.preset(1, 0.05))
+    k, B = 4, 4
+    W0, H0 = lib.init_factors(tr.m, tr.n, k, 2, 11)
+    rb = np.array([0, 3, 3, tr.m // 2, tr.m], np.int32)
+    Wo, Ho = O.sgd_strat(tr, W0, H0, storage=O.FP64, arith=O.ARITH_REF, B=B, epochs=1, row_bounds=rb, **HYP)
+    W, H = W0.copy(), H0.copy()
+    cpb = -(-tr.n // B)
+    eta = HYP["lrate"] * 1.5 / (1.0 + HYP["drate"])
+    for s in range(B):
+        for b in range(B):
+            cb = (b + s) % B
+            for j in range(cb * cpb, min(tr.n, (cb + 1) * cpb)):
+                for o in range(tr.colptr[j], tr.colptr[j + 1]):
+                    i = tr.rowidx[o]
+                    if not (rb[b] <= i < rb[b + 1]):
+                        continue
+                    err = W[i] @ H[j] - tr.val[o]
+                    wi = W[i].copy()
+                    W[i] -= eta * (err * H[j] + HYP["reg"] * wi)
+                    H[j] -= eta * (err * wi + HYP["reg"] * H[j])
+    assert np.allclose(Wo, W, rtol=1e-12, atol=1e-14) and np.allclose(Ho, H, rtol=1e-12, atol=1e-14)
+    # a different split is a different schedule
+    Wd, _ = O.sgd_strat(tr, W0, H0, storage=O.FP64, arith=O.ARITH_REF, B=B, epochs=1, **HYP)
+    assert not np.array_equal(Wd, Wo)
+    rpb = -(-tr.m // B)
+    eq = np.minimum(np.arange(B + 1) * rpb, tr.m).astype(np.int32)
+    for st in (O.FP32, O.BF16):
+        a = O.sgd_strat(tr, W0, H0, storage=st, arith=O.ARITH_GPU, B=B, epochs=2, row_bounds=eq, **HYP)
+        d = O.sgd_strat(tr, W0, H0, storage=st, arith=O.ARITH_GPU, B=B, epochs=2, **HYP)
+        assert np.array_equal(a[0], d[0]) and np.array_equal(a[1], d[1])
+    for bad in ([1, 3, 3, 5, tr.m], [0, 3, 2, 5, tr.m], [0, 3, 3, 5, tr.m - 1]):
+        with pytest.raises(RuntimeError):
+            O.sgd_strat(tr, W0, H0, storage=O.FP64, arith=O.ARITH_REF, B=B, epochs=1,
+                        row_bounds=np.array(bad, np.int32), **HYP)
+
+
 def test_gpu_order_close_to_reference_order():
     tr, _ = lib.synth(lib.preset(1, 0.2))
     k = 16
