@@ -137,11 +137,18 @@ __device__ __align__(16) static double g_zero_pairs[2 * kWave];  // zero-initial
 // (stores) or read as zero (loads) by the hardware's range check, so lanes
 // beyond the row need neither a branch nor an address select, and the
 // compiler counts every load and store exactly when it places its waits.
-// The row address must be wave-uniform (it lives in scalar registers).
+// The row address must be wave-uniform (it lives in scalar registers): it
+// and the size are read from the first lane, since a descriptor the
+// compiler builds in vector registers (from values it cannot prove
+// uniform) makes it wrap every access in a loop over distinct values.
 typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void* row, int bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(row), 0, bytes, 0x00020000);
+    const uint64_t v = reinterpret_cast<uint64_t>(row);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(int(uint32_t(v)));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(int(uint32_t(v >> 32)));
+    return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>((uint64_t(hi) << 32) | lo), 0,
+                                             __builtin_amdgcn_readfirstlane(bytes), 0x00020000);
 }
 
 // Cache-policy operand of a buffer access that makes it device-coherent:

@@ -76,13 +76,27 @@ struct RowBlock {
     __device__ __forceinline__ voff_t drop(int d = 0) const { return Store<S>::drop_voff(wvoff, d); }
 };
 
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(int(uint32_t(v)));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(int(uint32_t(v >> 32)));
+    return (uint64_t(hi) << 32) | lo;
+}
+
+// (the descriptor's fields are read from the first lane: they are uniform,
+// but where the arguments were loaded from memory into vector registers the
+// descriptor would otherwise be built in vector registers too, and the
+// compiler then wraps every buffer access in a loop over its distinct
+// values -- twelve instructions and an exec-mask round trip per access)
 template <int S>
 __device__ __forceinline__ RowBlock<S> row_block(const SgdArgs& a, int32_t rlo, int32_t rhi, int lane) {
     using T = Store<S>;
+    using E = typename T::elem;
     RowBlock<S> rb;
-    rb.rowbytes = a.ldf * int(sizeof(typename T::elem));
-    rb.rsrc = __builtin_amdgcn_make_buffer_rsrc(static_cast<typename T::elem*>(a.W) + int64_t(rlo) * a.ldf, 0,
-                                                (rhi - rlo) * rb.rowbytes, 0x00020000);
+    rlo = __builtin_amdgcn_readfirstlane(rlo);
+    rb.rowbytes = __builtin_amdgcn_readfirstlane(a.ldf * int(sizeof(E)));
+    const int32_t bytes = __builtin_amdgcn_readfirstlane((rhi - rlo) * rb.rowbytes);
+    const uint64_t base = uniform64(reinterpret_cast<uint64_t>(static_cast<E*>(a.W)) + uint64_t(rlo) * rb.rowbytes);
+    rb.rsrc = __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>(base), 0, bytes, 0x00020000);
     rb.wvoff = T::lane_voff(lane, a.ldf >> 1);
     rb.rlo = rlo;
     return rb;
@@ -246,12 +260,6 @@ __device__ __forceinline__ void run_visit_lds(const SgdArgs& a, typename Store<S
         wait_vm();
         run_chunk_lds<S, RM>(a, wl, npairs, h, myrow, myslot, myval, cnt, step, reg, sr_b, lkey);
     }
-}
-
-__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
-    const uint32_t lo = __builtin_amdgcn_readfirstlane(int(uint32_t(v)));
-    const uint32_t hi = __builtin_amdgcn_readfirstlane(int(uint32_t(v >> 32)));
-    return (uint64_t(hi) << 32) | lo;
 }
 
 }  // namespace nomad_dev
