@@ -1208,10 +1208,18 @@ int train_batch(nomad_ctx* const* cs, int32_t count, int64_t epochs, int32_t wav
     }
     // sorted by ticket rounds descending (those still running in a ticket
     // round are a prefix), then by the midpoint (2j + 1) / 2s of the entry's
-    // share of a ticket round, ascending
-    std::stable_sort(ent.begin(), ent.end(), [](const Entry& x, const Entry& y) {
+    // share of a ticket round, ascending, then by ratings per row block,
+    // ascending: a round's light tiles go first and free their waves for
+    // the next round's while the heavy ones run (measured on a sweep of 32
+    // problems: 40.0 ms per epoch light first, 40.7 in problem order, 44.4
+    // heavy first)
+    std::vector<double> heavy(static_cast<size_t>(count));
+    for (int32_t i = 0; i < count; ++i) heavy[size_t(i)] = double(cs[i]->tr.nnz) / double(cs[i]->P);
+    std::stable_sort(ent.begin(), ent.end(), [&](const Entry& x, const Entry& y) {
         if (x.ur != y.ur) return x.ur > y.ur;
-        return int64_t(2 * x.j + 1) * y.s < int64_t(2 * y.j + 1) * x.s;
+        const int64_t kx = int64_t(2 * x.j + 1) * y.s, ky = int64_t(2 * y.j + 1) * x.s;
+        if (kx != ky) return kx < ky;
+        return heavy[size_t(x.p)] < heavy[size_t(y.p)];
     });
     const int32_t E = int32_t(ent.size());
     std::vector<int32_t> order(static_cast<size_t>(E)), boff(static_cast<size_t>(E) + 1, 0);

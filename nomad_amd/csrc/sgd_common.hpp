@@ -195,20 +195,17 @@ __device__ __forceinline__ void run_visit_pre(const SgdArgs& a, const RowBlock<S
 constexpr int kTileLdsBytes = 8192;  // per wave
 constexpr int kStageRows = 256;      // most rows of W a tile stages in LDS
 
-template <int S>
-__device__ __forceinline__ typename Store<S>::raw lds_row_load(const typename Store<S>::raw* wl, int32_t off,
-                                                               bool inr) {
-    typename Store<S>::raw z{};
-    return inr ? wl[off] : z;
-}
-
 // run_chunk with the rows in LDS, rating t's row at slot myslot[t]: the same
 // steps in the same order (a column's rows are distinct, so a row loaded
-// ahead is never stale), the stored forms written back to LDS.
+// ahead is never stale), the stored forms written back to LDS.  Lanes past
+// the row (lane >= npairs) read and write the zero entry wl[zslot] instead
+// of a branch around the access: they hold h = 0, so they read 0 and write
+// back a zero (of either sign), and the loop body stays one basic block
+// (the scalar rounding hash then fills the reduction's wait slots).
 template <int S, int RM>
 __device__ __forceinline__ void run_chunk_lds(const SgdArgs& a, typename Store<S>::raw* wl, int npairs,
-                                              typename Store<S>::pair& h, int32_t myrow, int32_t myslot,
-                                              typename Store<S>::rval myval, int32_t cnt,
+                                              int32_t zslot, typename Store<S>::pair& h, int32_t myrow,
+                                              int32_t myslot, typename Store<S>::rval myval, int32_t cnt,
                                               typename Store<S>::acc step, typename Store<S>::acc reg,
                                               uint32_t sr_b, uint32_t lkey) {
     using T = Store<S>;
@@ -217,10 +214,10 @@ __device__ __forceinline__ void run_chunk_lds(const SgdArgs& a, typename Store<S
     constexpr int kD = 4;
     const int lane = lane_id();
     const bool inr = lane < npairs;
+    auto at = [&](int32_t slot) { return inr ? slot * npairs + lane : zslot; };
     typename T::raw wp[kD];
 #pragma unroll
-    for (int d = 0; d < kD; ++d)
-        wp[d] = lds_row_load<S>(wl, readlane_i32(myslot, min(d, cnt - 1)) * npairs + lane, inr);
+    for (int d = 0; d < kD; ++d) wp[d] = wl[at(readlane_i32(myslot, min(d, cnt - 1)))];
     for (int32_t t0 = 0;; t0 += kD) {
 #pragma unroll
         for (int d = 0; d < kD; ++d) {
@@ -230,8 +227,8 @@ __device__ __forceinline__ void run_chunk_lds(const SgdArgs& a, typename Store<S
             P w = T::unpack(wp[d]);
             sgd_step(w, h, Aa(readlane_val(myval, t)), step, reg);
             const uint32_t rnd = sr_bits_k(a.sr_seed, uint32_t(i), sr_b, lkey);
-            if (inr) wl[readlane_i32(myslot, t) * npairs + lane] = T::pack(w, rnd, RM);
-            wp[d] = lds_row_load<S>(wl, readlane_i32(myslot, min(t + kD, cnt - 1)) * npairs + lane, inr);
+            wl[at(readlane_i32(myslot, t))] = T::pack(w, rnd, RM);
+            wp[d] = wl[at(readlane_i32(myslot, min(t + kD, cnt - 1)))];
         }
     }
 chunk_done:;
@@ -239,7 +236,7 @@ chunk_done:;
 
 template <int S, int RM>
 __device__ __forceinline__ void run_visit_lds(const SgdArgs& a, typename Store<S>::raw* wl, int npairs,
-                                              typename Store<S>::pair& h, int32_t j, int64_t start, int32_t len,
+                                              int32_t zslot, typename Store<S>::pair& h, int32_t j, int64_t start, int32_t len,
                                               int32_t row0, int32_t slot0, typename Store<S>::rval val0,
                                               typename Store<S>::acc step, typename Store<S>::acc reg,
                                               uint32_t count, uint32_t lkey) {
@@ -249,7 +246,7 @@ __device__ __forceinline__ void run_visit_lds(const SgdArgs& a, typename Store<S
     const gptr<R> rv = gbl(static_cast<const R*>(a.val)) + start;
     const gptr<uint16_t> sl = gbl(a.tslot) + start;
     const uint32_t sr_b = uint32_t(j) * 0x01000193u + count;
-    run_chunk_lds<S, RM>(a, wl, npairs, h, row0, slot0, val0, min(kWave, len), step, reg, sr_b, lkey);
+    run_chunk_lds<S, RM>(a, wl, npairs, zslot, h, row0, slot0, val0, min(kWave, len), step, reg, sr_b, lkey);
     for (int32_t base = kWave; base < len; base += kWave) {
         const int32_t cnt = min(kWave, len - base);
         const int32_t myrow = ridx[base + min(lane, cnt - 1)];
@@ -258,7 +255,7 @@ __device__ __forceinline__ void run_visit_lds(const SgdArgs& a, typename Store<S
         // (the chunk loop is shared with the first chunk, whose ratings come
         // from LDS: waiting here keeps its waits off the first chunk)
         wait_vm();
-        run_chunk_lds<S, RM>(a, wl, npairs, h, myrow, myslot, myval, cnt, step, reg, sr_b, lkey);
+        run_chunk_lds<S, RM>(a, wl, npairs, zslot, h, myrow, myslot, myval, cnt, step, reg, sr_b, lkey);
     }
 }
 

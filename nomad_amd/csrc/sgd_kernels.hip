@@ -109,7 +109,8 @@ __device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep,
     const RowBlock<S> rb = row_block<S>(a, rlo, rhi, lane);
     const uint32_t lkey = sr_lane_key(a.sr_seed, uint32_t(lane));
     const int32_t staged = nrows * npairs;
-    const bool in_lds = T::kNP == 1 && nrows <= kStageRows && staged <= TL::kW;
+    // (the last entry of the LDS copy stays zero: the lanes past a row use it)
+    const bool in_lds = T::kNP == 1 && nrows <= kStageRows && staged < TL::kW;
     constexpr int CP = DC ? kCpDevice : 0;
 
     // Group descriptors: lane u <= kK holds visit f + u (clamped to v1, the
@@ -230,6 +231,7 @@ __device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep,
                 if (u * kWave >= staged) break;
                 if (u * kWave + lane < staged) L.w[u * kWave + lane] = t[u];
             }
+            if (lane == 0) L.w[TL::kW - 1] = raw{};
         }
         // groups g (dc, from fc, nc visits), g + 1 (d1, f1, n1), g + 2 (d2, f2)
         int64_t fc = v0;
@@ -263,7 +265,8 @@ __device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep,
                 const R val = G.val[o];
                 if constexpr (LW) {
                     const int32_t row = RM == NOMAD_ROUND_STOCHASTIC ? row_of_slot(key) : 0;
-                    run_visit_lds<S, RM>(a, L.w, npairs, h, j, st, ln, row, key, val, step, reg, uint32_t(epoch),
+                    run_visit_lds<S, RM>(a, L.w, npairs, TL::kW - 1, h, j, st, ln, row, key, val, step, reg,
+                                         uint32_t(epoch),
                                          lkey);
                 } else {
                     run_visit_pre<S, DC, RM>(a, rb, h, j, st, ln, key, val, step, reg, uint32_t(epoch), lkey);
