@@ -95,6 +95,42 @@ __global__ void ccd_rank_update(int64_t nnz, const int32_t* rowidx, const int32_
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
+// Ranks that stop at their first inner iteration before an outer iteration
+// ends early (Code/competitor/ccdr1-mpi-omp.cpp:432).
+constexpr int kEarlyStopRanks = 5;
+
+// Rank t's term out and the next rank's back in, in one pass:
+//   x = r[e] - u[row] v[col] ; r[e] = x + u_n[row] v_n[col]
+// with the two roundings of two separate ccd_rank_update passes (each line
+// is that kernel's expression), and with part != nullptr the sum of x^2 per
+// block (the loss after rank t).  The next rank is t + 1 (u1, v1) while the
+// device's early-stop count is below the limit, else rank 0 of the next
+// outer iteration (u0, v0; null: none).
+__global__ void ccd_rank_swap(int64_t nnz, const int32_t* rowidx, const int32_t* ecol, const double* u,
+                              const double* v, const double* u1, const double* v1, const double* u0,
+                              const double* v0, const CcdState* st, double* r, double* part) {
+    __shared__ double s[256];
+    const bool go1 = u1 && st->early_stop < kEarlyStopRanks;
+    const double* un = go1 ? u1 : u0;
+    const double* vn = go1 ? v1 : v0;
+    const double sign = -1.0, sign_n = 1.0;
+    double acc = 0;
+    for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < nnz; e += int64_t(gridDim.x) * blockDim.x) {
+        const int32_t i = rowidx[e], j = ecol[e];
+        const double x = r[e] + sign * (u[i] * v[j]);
+        acc += x * x;
+        r[e] = un ? x + sign_n * (un[i] * vn[j]) : x;
+    }
+    if (!part) return;
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
 // the CSR-ordered copy of the residual: rt[d] = r[perm[d]] (once, at set-up)
 __global__ void ccd_gather(int64_t nnz, const int32_t* perm, const double* r, double* rt) {
     for (int64_t d = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; d < nnz; d += int64_t(gridDim.x) * blockDim.x)
@@ -404,6 +440,15 @@ extern "C" int nomad_ccd(const nomad_ccd_params* p, const int64_t* colptr, const
             hipLaunchKernelGGL(ccd_sum_stage2, dim3(1), dim3(256), 0, s, kRedBlocks, d_part, d_out, slot,
                                static_cast<const CcdState*>(nullptr));
     };
+    auto rank_swap = [&](int64_t count, const int32_t* ri, const int32_t* ci, const double* u, const double* v,
+                         const double* u1, const double* v1, const double* u0, const double* v0, double* r,
+                         bool want_loss, int slot) {
+        hipLaunchKernelGGL(ccd_rank_swap, dim3(kRedBlocks), dim3(256), 0, s, count, ri, ci, u, v, u1, v1, u0, v0,
+                           static_cast<const CcdState*>(d_state), r, want_loss ? d_part : nullptr);
+        if (want_loss)
+            hipLaunchKernelGGL(ccd_sum_stage2, dim3(1), dim3(256), 0, s, kRedBlocks, d_part, d_out, slot,
+                               static_cast<const CcdState*>(nullptr));
+    };
     auto pass = [&](const Side& sd, const double* x, double* y) {
         hipLaunchKernelGGL(ccd_pass_partial, dim3(grid_for(int64_t(sd.ntasks) * 64, 256)), dim3(256), 0, s,
                            sd.tasks, sd.ntasks, sd.idx, sd.r, x, sd.part, d_state);
@@ -426,18 +471,26 @@ extern "C" int nomad_ccd(const nomad_ccd_params* p, const int64_t* colptr, const
     int64_t ranks_done = 0, inner_done = 0;
     if ((rc = fetch())) return rc;  // set-up work finished
     const auto t_solve = std::chrono::steady_clock::now();
+    // The residual passes of consecutive ranks are fused: rank t's term is
+    // taken out and the next rank's added back in one pass (ccd_rank_swap),
+    // which decides on the device, from the same early-stop count the host
+    // reads afterwards, which rank is next; `added` says that the current
+    // rank's term is already back in the residuals.
+    bool added = false;
     for (int oiter = 1; oiter <= p->iters; ++oiter) {
         // fundec_max and the early-stop count restart with every outer iteration
         hipLaunchKernelGGL(ccd_state_reset, dim3(1), dim3(1), 0, s, d_state, 1);
         host_st.early_stop = 0;
         for (int t = 0; t < k; ++t) {
-            if (host_st.early_stop >= 5) break;
+            if (host_st.early_stop >= kEarlyStopRanks) break;
             double* u = d_Wt + size_t(t) * m;
             double* v = d_Ht + size_t(t) * n;
             hipLaunchKernelGGL(ccd_state_reset, dim3(1), dim3(1), 0, s, d_state, 0);
-            rank_update(nnz, d_rowidx, d_ecol, u, v, 1.0, d_r, false, 0);
-            rank_update(nnz, d_erow, d_colidx, u, v, 1.0, d_rt, false, 0);
-            if (has_test) rank_update(te_nnz, d_te_rowidx, d_te_ecol, u, v, 1.0, d_te_r, false, 0);
+            if (!added) {
+                rank_update(nnz, d_rowidx, d_ecol, u, v, 1.0, d_r, false, 0);
+                rank_update(nnz, d_erow, d_colidx, u, v, 1.0, d_rt, false, 0);
+                if (has_test) rank_update(te_nnz, d_te_rowidx, d_te_ecol, u, v, 1.0, d_te_r, false, 0);
+            }
             sum_into(m, u, d_rowptr, 4);
             sum_into(n, v, d_colptr, 5);
             // all inner iterations queued at once; the device stops them
@@ -448,12 +501,20 @@ extern "C" int nomad_ccd(const nomad_ccd_params* p, const int64_t* colptr, const
                 sum_into(m, rs.fundec, nullptr, 1, d_state);
                 hipLaunchKernelGGL(ccd_inner_decide, dim3(1), dim3(1), 0, s, d_state, d_out, iter, oiter, t, p->eps);
             }
-            rank_update(nnz, d_rowidx, d_ecol, u, v, -1.0, d_r, true, 2);
-            rank_update(nnz, d_erow, d_colidx, u, v, -1.0, d_rt, false, 0);
-            if (has_test) rank_update(te_nnz, d_te_rowidx, d_te_ecol, u, v, -1.0, d_te_r, true, 3);
+            // next rank: t + 1 (if the early-stop count allows it), else rank 0
+            // of the next outer iteration, else none
+            const double* u1 = t + 1 < k ? d_Wt + size_t(t + 1) * m : nullptr;
+            const double* v1 = t + 1 < k ? d_Ht + size_t(t + 1) * n : nullptr;
+            const double* u0 = oiter < p->iters ? d_Wt : nullptr;
+            const double* v0 = oiter < p->iters ? d_Ht : nullptr;
+            rank_swap(nnz, d_rowidx, d_ecol, u, v, u1, v1, u0, v0, d_r, true, 2);
+            rank_swap(nnz, d_erow, d_colidx, u, v, u1, v1, u0, v0, d_rt, false, 0);
+            if (has_test) rank_swap(te_nnz, d_te_rowidx, d_te_ecol, u, v, u1, v1, u0, v0, d_te_r, true, 3);
             sum_into(m, u, d_rowptr, 0);
             sum_into(n, v, d_colptr, 1);
             if ((rc = fetch())) return rc;  // the one read-back of the rank
+            // (the swap's choice, repeated: a next rank's term is back in)
+            added = (t + 1 < k && host_st.early_stop < kEarlyStopRanks) || oiter < p->iters;
             inner_done = host_st.inner_done;
             reg -= host_out[4] + host_out[5];
             reg += host_out[0] + host_out[1];
@@ -477,10 +538,12 @@ extern "C" int nomad_ccd(const nomad_ccd_params* p, const int64_t* colptr, const
         st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
         st->solve_seconds = solve_s;
         // per inner iteration: both passes read index + residual of every
-        // rating and write both vectors; per rank: residual read + write for
-        // the add and the subtract, training and test
+        // rating and write both vectors; per rank: one residual read + write
+        // (the fused take-out and add-back), training (both orders) and test,
+        // plus the first rank's add-back
         st->alg_bytes = double(inner_done) * (2.0 * double(nnz) * 12.0 + double(m + n) * 16.0) +
-                        double(ranks_done) * 2.0 * double(2 * nnz + (has_test ? te_nnz : 0)) * 16.0;
+                        double(ranks_done + (ranks_done > 0 ? 1 : 0)) * double(2 * nnz + (has_test ? te_nnz : 0)) *
+                            16.0;
     }
     return NOMAD_OK;
 }

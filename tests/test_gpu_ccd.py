@@ -42,15 +42,22 @@ def test_ccd_matches_reference_trace():
     assert st.ranks_done == len(c["trace"])
 
 
-@pytest.mark.parametrize("k,lam", [(16, 0.05), (5, 0.5)])
-def test_ccd_matches_oracle(small, k, lam):
+@pytest.mark.parametrize("k,lam,eps,iters", [(16, 0.05, 1e-3, 2), (5, 0.5, 1e-3, 2), (16, 0.05, 0.3, 3)])
+def test_ccd_matches_oracle(small, k, lam, eps, iters):
+    """Values and decisions against the oracle; with eps = 0.3 ranks stop at
+    their first inner iteration, so outer iterations end early and the next
+    one starts at rank 0 (the residual passes of consecutive ranks are fused,
+    and the fused pass picks the next rank on the device)."""
     tr, te = small
     rng = np.random.default_rng(k)
     W0 = rng.random((tr.m, k)) / np.sqrt(k)
     H0 = np.zeros((tr.n, k))
-    print(f"[ccd] {tr.m}x{tr.n} k={k} lambda={lam}", flush=True)
-    Wg, Hg, og, rg, st = lib.ccd(tr, W0, H0, lam, 2, 3, 1e-3, test=te)
-    Wo, Ho, oo, ro = O.ccd(tr, W0, H0, lam, 2, 3, 1e-3, test=te)
+    print(f"[ccd] {tr.m}x{tr.n} k={k} lambda={lam} eps={eps} iters={iters}", flush=True)
+    Wg, Hg, og, rg, st = lib.ccd(tr, W0, H0, lam, iters, 3, eps, test=te)
+    Wo, Ho, oo, ro = O.ccd(tr, W0, H0, lam, iters, 3, eps, test=te)
+    if eps > 0.1:  # the early-stop path is exercised: an outer iteration ends early and the next one runs
+        skipped = np.isnan(oo).reshape(iters, k)
+        assert skipped[0].any() and not skipped[1:].all(axis=1).any(), skipped
     # same decisions (skipped ranks) and the same values up to FP64 summation order
     np.testing.assert_array_equal(np.isnan(og), np.isnan(oo))
     ok = ~np.isnan(oo)
