@@ -147,18 +147,19 @@ __global__ void ccd_pass_partial(const CcdTask* tasks, int ntasks, const int32_t
     for (int t = wave; t < ntasks; t += nwaves) {
         const CcdTask tk = tasks[t];
         double g = 0, h = 0;
-        // four of the lane's ratings in flight, added in the same order
+        // kU of the lane's ratings in flight, added in the same order
+        constexpr int kU = 8;
         int q = lane;
-        for (; q + 192 < tk.len; q += 256) {
-            double xv[4], rv[4];
+        for (; q + 64 * (kU - 1) < tk.len; q += 64 * kU) {
+            double xv[kU], rv[kU];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kU; ++u) {
                 const int64_t e = tk.e0 + q + 64 * u;
                 xv[u] = x[idx[e]];
                 rv[u] = r[e];
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kU; ++u) {
                 g += xv[u] * rv[u];
                 h += xv[u] * xv[u];
             }
