@@ -62,14 +62,16 @@ using nomad_k::VisitDesc;
 // rows' values, and the order in which the steps see them, are those of the
 // HBM path, so the result is the same bit for bit, but a step waits for LDS
 // instead of HBM.
-template <int S>
+template <int S, int WB = kTileLdsBytes>
 struct TileLds {
     using T = Store<S>;
     using raw = typename T::raw;
     using R = typename T::rval;
     static constexpr int kB = base_storage(S);
-    // staged W (the two-pair layout does not stage W: its rows take the HBM path)
-    static constexpr int kW = T::kNP == 1 ? kTileLdsBytes / int(sizeof(raw)) : kWave;
+    // staged W, WB bytes (the two-pair layout does not stage W: its rows take
+    // the HBM path); a multiple of 64 entries
+    static constexpr int kW = T::kNP == 1 ? WB / int(sizeof(raw)) : kWave;
+    static_assert(kW % kWave == 0, "the staged W copy is loaded 64 entries at a time");
     static constexpr int kK = kB == NOMAD_STORE_FP64 ? 4 : 8;  // visits per group
     static constexpr int kQ = kB == NOMAD_STORE_BF16 ? 4 : (kB == NOMAD_STORE_FP32 ? 2 : 1);  // x64 ratings
     struct Group {
@@ -81,10 +83,9 @@ struct TileLds {
     Group g[2];
 };
 
-template <int S, bool DC, int RM>
-__device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep, int epoch, TileLds<S>& L) {
+template <int S, bool DC, int RM, class TL>
+__device__ __forceinline__ void strat_tile(const SgdArgs& a, int b, int substep, int epoch, TL& L) {
     using T = Store<S>;
-    using TL = TileLds<S>;
     using E = typename T::elem;
     using Aa = typename T::acc;
     using R = typename T::rval;
@@ -348,6 +349,14 @@ __global__ __launch_bounds__(64) void sgd_strat_kernel(SgdArgs a) {
 // wave that sees the counter reads the new rows; no L2 writeback or
 // invalidate is needed.  Counters live in uncached memory and are polled
 // with device-coherent loads.
+// Staged W per wave in the batched kernel: bf16 takes the LDS that two
+// 4-wave workgroups per CU leave over (10,496 B instead of 8 KiB), so that
+// more tiles keep their rows in LDS.
+template <int S>
+constexpr int batch_stage_bytes() {
+    return S == NOMAD_STORE_BF16 ? 10496 : kTileLdsBytes;
+}
+
 template <int S>
 __global__ __launch_bounds__(256) void sgd_batch_kernel(BatchArgs ba) {
     __shared__ int64_t seg_t0[kBatchEntries];
@@ -355,7 +364,7 @@ __global__ __launch_bounds__(256) void sgd_batch_kernel(BatchArgs ba) {
     __shared__ int32_t seg_w[kBatchEntries];
     __shared__ int32_t boff[kBatchEntries + 1];
     __shared__ int32_t order[kBatchEntries];
-    __shared__ TileLds<S> tl[4];  // per wave (256-thread workgroups)
+    __shared__ TileLds<S, batch_stage_bytes<S>()> tl[4];  // per wave (256-thread workgroups)
     const int Q = ba.nsorted, G = ba.nseg;
     for (int i = threadIdx.x; i <= Q; i += blockDim.x) {
         if (i < G) {
