@@ -73,7 +73,7 @@ struct TileLds {
     static constexpr int kW = T::kNP == 1 ? WB / int(sizeof(raw)) : kWave;
     static_assert(kW % kWave == 0, "the staged W copy is loaded 64 entries at a time");
     static constexpr int kK = kB == NOMAD_STORE_FP64 ? 4 : 8;  // visits per group
-    static constexpr int kQ = kB == NOMAD_STORE_BF16 ? 4 : (kB == NOMAD_STORE_FP32 ? 2 : 1);  // x64 ratings
+    static constexpr int kQ = kB == NOMAD_STORE_BF16 ? (WB > kTileLdsBytes ? 3 : 4) : (kB == NOMAD_STORE_FP32 ? 2 : 1);  // x64 ratings
     struct Group {
         raw h[kK * kWave];      // visit u's h pair of lane l at [u * 64 + l]
         int32_t key[kQ * kWave];  // rating's LDS slot (W staged) or row
@@ -350,11 +350,12 @@ __global__ __launch_bounds__(64) void sgd_strat_kernel(SgdArgs a) {
 // invalidate is needed.  Counters live in uncached memory and are polled
 // with device-coherent loads.
 // Staged W per wave in the batched kernel: bf16 takes the LDS that two
-// 4-wave workgroups per CU leave over (10,496 B instead of 8 KiB), so that
-// more tiles keep their rows in LDS.
+// 4-wave workgroups per CU leave over, and 1 KiB more from staging 3 x 64
+// ratings per visit group instead of 4 x 64 (11,520 B instead of 8 KiB),
+// so that more tiles keep their rows in LDS.
 template <int S>
 constexpr int batch_stage_bytes() {
-    return S == NOMAD_STORE_BF16 ? 10496 : kTileLdsBytes;
+    return S == NOMAD_STORE_BF16 ? 11520 : kTileLdsBytes;
 }
 
 template <int S>
