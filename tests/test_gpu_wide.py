@@ -68,6 +68,33 @@ def test_wide_strat_bf16_bitexact(gpu, tiny, rounding):
     assert np.array_equal(Wg, Wo) and np.array_equal(Hg, Ho)
 
 
+@pytest.mark.parametrize("storage", ["fp32", "bf16"])
+def test_wide_strat_balanced_rows_bitexact(gpu, tiny, storage):
+    """Row blocks balanced by ratings (NOMAD_FLAG_BALANCE) with two element
+    pairs per lane: single and batched runs equal the oracle's replay with
+    the same row bounds."""
+    tr, te = tiny
+    k, B, seed = 210, 5, 99
+    rb = gpu.row_blocks(tr.rowidx, tr.m, B)
+    assert not np.array_equal(rb, np.minimum(np.arange(B + 1) * -(-tr.m // B), tr.m))
+    W0, H0 = gpu.init_factors(tr.m, tr.n, k, 4, seed)
+    rnd = "stochastic" if storage == "bf16" else "nearest"
+    a = _ctx(gpu, tr, te, k, storage=storage, mode="strat", nworkers=B, balance=True, rounding=rnd, seed=seed)
+    b = _ctx(gpu, tr, te, k, storage=storage, mode="strat", nworkers=B, balance=True, rounding=rnd, seed=seed)
+    for c in (a, b):
+        c.set_factors(W0, H0)
+    a.train(epochs=2)
+    gpu.train_batch([b], 2)
+    ost = O.BF16 if storage == "bf16" else O.FP32
+    orn = O.ROUND_STOCHASTIC if storage == "bf16" else O.ROUND_NEAREST
+    Wo, Ho = O.sgd_strat(tr, W0, H0, storage=ost, arith=O.ARITH_GPU, rounding=orn, sr_seed=SR_SEED(seed), B=B,
+                         epochs=2, row_bounds=rb, **HYP)
+    for c, name in ((a, "single"), (b, "batched")):
+        Wg, Hg = c.get_factors()
+        assert np.array_equal(Wg, Wo) and np.array_equal(Hg, Ho), (name, np.abs(Wg - Wo).max())
+        c.close()
+
+
 def test_wide_batch_equals_single(gpu, tiny):
     """Two wide problems (k 150 and 256) in one batched launch, each equal bit
     for bit to its own run."""
