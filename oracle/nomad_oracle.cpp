@@ -17,7 +17,8 @@
 //   orc_sgd_strat    ARITH_REF: nb_main.cpp:514-544 restated; pinned by a
 //                    hand-computed one-rating update (k = 2) and by an
 //                    independent numpy FP64 replay of the serial order on small
-//                    inputs.  ARITH_GPU: the same algorithm with the GPU's
+//                    inputs (also of a B = 4 tile order, with the default and
+//                    with uneven given row bounds).  ARITH_GPU: the same algorithm with the GPU's
 //                    fused update and 64-lane tree; pinned against ARITH_REF
 //                    (agreement to rounding) and closed forms.
 //   orc_sgd_async    nb_main.cpp:415-578; pinned by invariants (update count =
