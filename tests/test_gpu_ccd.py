@@ -42,12 +42,14 @@ def test_ccd_matches_reference_trace():
     assert st.ranks_done == len(c["trace"])
 
 
-@pytest.mark.parametrize("k,lam,eps,iters", [(16, 0.05, 1e-3, 2), (5, 0.5, 1e-3, 2), (16, 0.05, 0.3, 3)])
+@pytest.mark.parametrize("k,lam,eps,iters", [(16, 0.05, 1e-3, 2), (5, 0.5, 1e-3, 2), (16, 0.05, 0.3, 3),
+                                            (1, 0.05, 1e-3, 3)])
 def test_ccd_matches_oracle(small, k, lam, eps, iters):
     """Values and decisions against the oracle; with eps = 0.3 ranks stop at
     their first inner iteration, so outer iterations end early and the next
     one starts at rank 0 (the residual passes of consecutive ranks are fused,
-    and the fused pass picks the next rank on the device)."""
+    and the fused pass picks the next rank on the device); with k = 1 every
+    fused pass hands over to rank 0 of the next outer iteration."""
     tr, te = small
     rng = np.random.default_rng(k)
     W0 = rng.random((tr.m, k)) / np.sqrt(k)
